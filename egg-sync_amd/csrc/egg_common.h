@@ -1,0 +1,132 @@
+// Shared definitions of the EGG-SynC HIP implementation (gfx950).
+//
+// Layout in HBM (one solve, n points, d dims, C non-empty cells, O outer cells):
+//   X[d][n]            float  positions, structure-of-arrays, in the order of
+//                             the last grid build (state between iterations)
+//   Xs[d][n]           float  positions sorted by cell (this iteration)
+//   perm[n], perm_s[n] int32  point id at each position
+//   keys[KW][n]        u64    cell key (outer id, g_od, ..., g_{d-1}) bit fields
+//   cell_start[C+1], cell_g[d][n], cell_outer[n], Ts[d][n], Tc[d][n]
+//   ob[O], oe[O]       int32  cell range of each outer cell
+// All FP arithmetic that decides membership is done exactly as the
+// reference's source reads (no FMA contraction: the library is built with
+// -ffp-contract=off), so the integer results equal the CPU oracle's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace egg {
+
+constexpr int kMaxD = 8;          // dimensions supported by the device kernels
+constexpr int kBlock = 256;       // threads per block (4 wavefronts)
+constexpr int kWave = 64;
+constexpr int kScanItems = 8;     // items per thread in scans
+constexpr int kScanTile = kBlock * kScanItems;
+constexpr int kSortItems = 8;     // items per thread in radix passes
+constexpr int kSortTile = kBlock * kSortItems;
+constexpr int kSumChunk = 2048;   // max points per summary work item
+
+// Grid geometry and thresholds, computed on the host (EGG_SynC.cu:1249-1260).
+struct Geo {
+    int n;
+    int d;
+    float eps;
+    float cs;          // cell side
+    float eps_extra;   // EGG_SynC.cu:1430
+    int w;             // cells per dimension
+    int od;            // outer dimensions
+    int O;             // outer cells (w^od <= n)
+    int R;             // ceil(eps/cs)
+    int R_extra;       // ceil(eps_extra/cs)
+    int R_half;        // ceil((eps/2.)/cs)
+    float thr_eps;     // largest s with sqrtf(s) <= eps
+    float thr_half;    // largest s with sqrtf(s) <= eps/2
+    float thr_extra;   // largest s with sqrtf(s) <= eps_extra
+    int bits_w;        // bits of one coordinate field in the key
+    int bits_o;        // bits of the outer id field
+    int key_bits;      // total key bits
+    int kw;            // 64-bit key words (1 or 2)
+    int wpow[kMaxD];   // w^l, l < od
+};
+
+// Device-resident loop state (one per solve).
+struct DevState {
+    int done;          // 1: converged or capped; later kernels of the loop are no-ops
+    int iter;          // iterations completed
+    int flag;          // local-sync flag of the current iteration (1 = synced)
+    int flag_local;    // flag before the extra check (diagnostics)
+    int C;             // non-empty cells of the current grid
+    int W;             // summary work items
+    int capped;        // 1 if max_iter stopped the loop
+    int max_iter;      // 0 = unlimited
+    int stop_rule;     // 0: EGG local sync + extra check; 1: r_c >= lam
+    int n_clusters;
+    float lam;
+    int pad;
+    double rc;         // order parameter of the last r_c evaluation
+};
+
+// ---------------------------------------------------------------- helpers --
+
+// compute_cell_id per coordinate (EGG_SynC.cu:67-74): FP32 division, truncation
+// toward zero, w clamped to w-1 (other out-of-range values clamped too).
+__device__ __forceinline__ int cell_coord(float x, float cs, int w) {
+    float q = x / cs;
+    if (!(q > -1.0f)) return 0;
+    if (q >= (float)w) return w - 1;
+    int t = (int)q;
+    return t >= w ? w - 1 : t;
+}
+
+// On-border test (EGG_SynC.cu:590-595), in double like the reference.
+__device__ __forceinline__ bool on_border(float x, float cs) {
+    double xd = (double)x;
+    int idm = (int)((xd - 0.00001) / (double)cs);
+    int id = (int)(x / cs);
+    int idp = (int)((xd + 0.00001) / (double)cs);
+    return id != idm || id != idp;
+}
+
+// Box test of a point against an inner cell (EGG_SynC.cu:612-634): squared
+// per-dimension distances to the two faces, the smaller summed into `small`,
+// the larger into `large`.  This is the reference's formula, including its
+// treatment of dimensions where x lies inside the cell's extent.
+template <int D>
+__device__ __forceinline__ void box_dist(const int *g, const float *x, float cs, float &small,
+                                         float &large) {
+    float s = 0.f, L = 0.f;
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        float lo = (float)g[l] * cs;
+        float hi = (float)(g[l] + 1) * cs;
+        lo -= x[l];
+        hi -= x[l];
+        lo *= lo;
+        hi *= hi;
+        if (lo < hi) {
+            s += lo;
+            L += hi;
+        } else {
+            s += hi;
+            L += lo;
+        }
+    }
+    small = s;
+    large = L;
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    return (1ull << __lane_id()) - 1ull;
+}
+
+// Kahan-Babuska combination of two compensated sums (s, c).
+__device__ __forceinline__ void two_sum_add(float &s, float &c, float s2, float c2) {
+    float t = s + s2;
+    float bp = t - s;
+    float err = (s - (t - bp)) + (s2 - bp);
+    s = t;
+    c = c + c2 + err;
+}
+
+}  // namespace egg

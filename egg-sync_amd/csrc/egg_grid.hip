@@ -1,0 +1,362 @@
+// Grid build and per-cell summaries (EGG_SynC.cu:248-366), gfx950.
+//
+// build_the_grid of the reference (two-level counting sort with atomic slot
+// order and linear slot searches) becomes: per-point bit-field cell keys ->
+// stable radix sort -> gather of positions into cell-sorted SoA order -> head
+// flags + exclusive scan -> cell table -> outer-cell ranges.  Cells and the
+// points within a cell come out in a deterministic order.
+//
+// kernel_inner_grid_stats (float atomics) becomes a deterministic wave-level
+// Kahan-compensated reduction per cell chunk (<= 2048 points), with a second
+// pass combining the chunks of large cells in order.
+#include "egg_internal.h"
+
+namespace egg {
+
+__global__ void k_aos_to_soa(const float *__restrict__ aos, float *__restrict__ soa,
+                             int *__restrict__ perm, int n, int d) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        for (int l = 0; l < d; l++) soa[(long long)l * n + i] = aos[i * d + l];
+        if (perm) perm[i] = (int)i;
+    }
+}
+
+__global__ void k_soa_to_aos_by_perm(const float *__restrict__ soa, const int *__restrict__ perm,
+                                     float *__restrict__ aos, int n, int d) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long p = perm[i];
+        for (int l = 0; l < d; l++) aos[p * d + l] = soa[(long long)l * n + i];
+    }
+}
+
+__global__ void k_fill(int *__restrict__ p, int count, int v, const DevState *st) {
+    if (st && st->done) return;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static inline int grid_for(long long n, int per_block = kBlock, int cap = 65536) {
+    long long b = (n + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+void launch_aos_to_soa(const float *aos, float *soa, int *perm, int n, int d, hipStream_t s) {
+    k_aos_to_soa<<<grid_for(n), kBlock, 0, s>>>(aos, soa, perm, n, d);
+}
+
+void launch_soa_to_aos_by_perm(const float *soa, const int *perm, float *aos, int n, int d,
+                               hipStream_t s) {
+    k_soa_to_aos_by_perm<<<grid_for(n), kBlock, 0, s>>>(soa, perm, aos, n, d);
+}
+
+void launch_fill(int *p, int count, int v, const DevState *st, hipStream_t s) {
+    if (count > 0) k_fill<<<grid_for(count), kBlock, 0, s>>>(p, count, v, st);
+}
+
+// Key bit fields: [outer | g_od | ... | g_{d-1}], g_{d-1} least significant,
+// so numeric key order = lexicographic (outer, g_od, ..., g_{d-1}).
+__device__ __forceinline__ void put_bits(uint64_t &lo, uint64_t &hi, uint64_t v, int shift,
+                                         int width) {
+    if (shift >= 64) {
+        hi |= v << (shift - 64);
+    } else {
+        lo |= v << shift;
+        if (shift + width > 64) hi |= v >> (64 - shift);
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_keys(const float *__restrict__ X, Geo g,
+                                                 uint64_t *__restrict__ keys, DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int n = g.n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int c[D];
+#pragma unroll
+        for (int l = 0; l < D; l++) c[l] = cell_coord(X[(long long)l * n + i], g.cs, g.w);
+        uint64_t lo = 0, hi = 0;
+        int shift = 0;
+#pragma unroll
+        for (int l = D - 1; l >= 0; l--) {
+            if (l >= g.od) {
+                put_bits(lo, hi, (uint64_t)c[l], shift, g.bits_w);
+                shift += g.bits_w;
+            }
+        }
+        uint64_t outer = 0;
+#pragma unroll
+        for (int l = 0; l < D; l++)
+            if (l < g.od) outer += (uint64_t)c[l] * (uint64_t)g.wpow[l];
+        put_bits(lo, hi, outer, shift, g.bits_o);
+        keys[i] = lo;
+        if (g.kw == 2) keys[(long long)n + i] = hi;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_gather(
+    const float *__restrict__ X, const int *__restrict__ perm, const uint64_t *__restrict__ skeys,
+    const int *__restrict__ sidx, Geo g, float *__restrict__ Xs, int *__restrict__ perm_s,
+    int *__restrict__ head, DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int n = g.n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int j = sidx ? sidx[i] : i;
+#pragma unroll
+        for (int l = 0; l < D; l++) Xs[(long long)l * n + i] = X[(long long)l * n + j];
+        perm_s[i] = perm[j];
+        int h = 1;
+        if (i > 0) {
+            h = skeys[i] != skeys[i - 1];
+            if (g.kw == 2) h |= skeys[(long long)n + i] != skeys[(long long)n + i - 1];
+        }
+        head[i] = h;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_cells(const float *__restrict__ Xs,
+                                                  const int *__restrict__ head,
+                                                  const int *__restrict__ hpos, Geo g,
+                                                  int *__restrict__ point_cell,
+                                                  int *__restrict__ cell_start,
+                                                  int *__restrict__ cell_g,
+                                                  int *__restrict__ cell_outer, DevState *st_loop,
+                                                  DevState *st) {
+    if (st_loop && st_loop->done) return;
+    const int n = g.n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int h = head[i];
+        const int c = hpos[i] + h - 1;
+        point_cell[i] = c;
+        if (h) {
+            cell_start[c] = i;
+            int outer = 0;
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                const int gl = cell_coord(Xs[(long long)l * n + i], g.cs, g.w);
+                cell_g[(long long)l * n + c] = gl;
+                if (l < g.od) outer += gl * g.wpow[l];
+            }
+            cell_outer[c] = outer;
+        }
+        if (i == n - 1) {
+            cell_start[c + 1] = n;
+            st->C = c + 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_outer_ranges(const int *__restrict__ cell_outer,
+                                                         const DevState *st,
+                                                         int *__restrict__ ob,
+                                                         int *__restrict__ oe,
+                                                         DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int C = st->C;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+        const int o = cell_outer[c];
+        if (c == 0 || cell_outer[c - 1] != o) ob[o] = c;
+        if (c == C - 1 || cell_outer[c + 1] != o) oe[o] = c + 1;
+    }
+}
+
+void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
+    const int n = g.n;
+    const int gb = grid_for(n);
+#define KEYS_CASE(DD) \
+    case DD: k_keys<DD><<<gb, kBlock, 0, s>>>(b.X, g, b.keys_a, st_loop); break;
+    switch (g.d) {
+        KEYS_CASE(1) KEYS_CASE(2) KEYS_CASE(3) KEYS_CASE(4)
+        KEYS_CASE(5) KEYS_CASE(6) KEYS_CASE(7) KEYS_CASE(8)
+    }
+#undef KEYS_CASE
+    int where = radix_sort(b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, g.kw, g.key_bits, b.hist,
+                           b.scan_tmp, st_loop, s);
+    const uint64_t *skeys = where == 1 ? b.keys_b : b.keys_a;
+    const int *sidx = where == 1 ? b.vals_b : (where == 0 ? b.vals_a : nullptr);
+#define GATHER_CASE(DD)                                                                  \
+    case DD:                                                                             \
+        k_gather<DD><<<gb, kBlock, 0, s>>>(b.X, b.perm, skeys, sidx, g, b.Xs, b.perm_s,  \
+                                           b.head, st_loop);                            \
+        break;
+    switch (g.d) {
+        GATHER_CASE(1) GATHER_CASE(2) GATHER_CASE(3) GATHER_CASE(4)
+        GATHER_CASE(5) GATHER_CASE(6) GATHER_CASE(7) GATHER_CASE(8)
+    }
+#undef GATHER_CASE
+    scan_exclusive(b.head, b.hpos, n, nullptr, b.scan_tmp, nullptr, st_loop, s);
+    launch_fill(b.ob, g.O, 0, st_loop, s);
+    launch_fill(b.oe, g.O, 0, st_loop, s);
+#define CELLS_CASE(DD)                                                                      \
+    case DD:                                                                                \
+        k_cells<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.head, b.hpos, g, b.point_cell,            \
+                                          b.cell_start, b.cell_g, b.cell_outer, st_loop, st); \
+        break;
+    switch (g.d) {
+        CELLS_CASE(1) CELLS_CASE(2) CELLS_CASE(3) CELLS_CASE(4)
+        CELLS_CASE(5) CELLS_CASE(6) CELLS_CASE(7) CELLS_CASE(8)
+    }
+#undef CELLS_CASE
+    k_outer_ranges<<<gb, kBlock, 0, s>>>(b.cell_outer, st, b.ob, b.oe, st_loop);
+}
+
+// ------------------------------------------------------------- summaries --
+
+__global__ __launch_bounds__(kBlock) void k_wi_count(const int *__restrict__ cell_start,
+                                                     const DevState *st,
+                                                     int *__restrict__ nchunk,
+                                                     DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int C = st->C;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+        const int cnt = cell_start[c + 1] - cell_start[c];
+        nchunk[c] = (cnt + kSumChunk - 1) / kSumChunk;
+    }
+}
+
+// One wavefront per work item (<= kSumChunk points of one cell).  Per dim:
+// S_sin = sum(sinf(x) - sinf(g*cs)), S_cos likewise (EGG_SynC.cu:362-363),
+// Kahan-compensated per lane, combined across lanes by compensated butterflies.
+// Single-chunk cells get T_sin = sinf(g*cs)*cnt + S_sin directly (the bracket
+// of EGG_SynC.cu:651-654); multi-chunk cells leave (sum, comp) partials.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_summary(
+    const float *__restrict__ Xs, const int *__restrict__ cell_start,
+    const int *__restrict__ cell_g, const int *__restrict__ wi_off,
+    const int *__restrict__ nchunk, Geo g, const DevState *st, float *__restrict__ Ts,
+    float *__restrict__ Tc, float *__restrict__ partial, DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int n = g.n, C = st->C, W = st->W;
+    const int lane = __lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    for (int wi = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; wi < W; wi += waves) {
+        int lo = 0, hi = C - 1;  // largest c with wi_off[c] <= wi
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (wi_off[mid] <= wi) lo = mid;
+            else hi = mid - 1;
+        }
+        const int c = lo;
+        const int k = wi - wi_off[c];
+        const int c0 = cell_start[c], c1 = cell_start[c + 1];
+        const int j0 = c0 + k * kSumChunk;
+        const int j1 = min(c1, j0 + kSumChunk);
+        const int nch = nchunk[c];
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            const float a = (float)cell_g[(long long)l * n + c] * g.cs;
+            const float sa = sinf(a), ca = cosf(a);
+            float ss = 0.f, sc_ = 0.f, cs_ = 0.f, cc = 0.f;  // (sum, comp) for sin and cos
+            for (int j = j0 + lane; j < j1; j += kWave) {
+                const float x = Xs[(long long)l * n + j];
+                const float ts = sinf(x) - sa;
+                const float tc = cosf(x) - ca;
+                float y = ts - sc_;
+                float t = ss + y;
+                sc_ = (t - ss) - y;
+                ss = t;
+                y = tc - cc;
+                t = cs_ + y;
+                cc = (t - cs_) - y;
+                cs_ = t;
+            }
+            // Kahan keeps sum - comp; switch to (sum, +err) form for the butterflies
+            float s1 = ss, e1 = -sc_, s2 = cs_, e2 = -cc;
+#pragma unroll
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                const float os1 = __shfl_xor(s1, o, kWave), oe1 = __shfl_xor(e1, o, kWave);
+                const float os2 = __shfl_xor(s2, o, kWave), oe2 = __shfl_xor(e2, o, kWave);
+                // combine in lane order so every lane holds the same bits
+                if (lane & o) {
+                    float a1 = os1, b1 = oe1, a2 = os2, b2 = oe2;
+                    two_sum_add(a1, b1, s1, e1);
+                    two_sum_add(a2, b2, s2, e2);
+                    s1 = a1; e1 = b1; s2 = a2; e2 = b2;
+                } else {
+                    two_sum_add(s1, e1, os1, oe1);
+                    two_sum_add(s2, e2, os2, oe2);
+                }
+            }
+            if (lane == 0) {
+                if (nch == 1) {
+                    const float cnt = (float)(c1 - c0);
+                    const float t1 = sa * cnt;
+                    const float t2 = ca * cnt;
+                    Ts[(long long)l * n + c] = t1 + (s1 + e1);
+                    Tc[(long long)l * n + c] = t2 + (s2 + e2);
+                } else {
+                    float *pp = &partial[((long long)wi * D + l) * 4];
+                    pp[0] = s1; pp[1] = e1; pp[2] = s2; pp[3] = e2;
+                }
+            }
+        }
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_summary_combine(
+    const int *__restrict__ cell_start, const int *__restrict__ cell_g,
+    const int *__restrict__ wi_off, const int *__restrict__ nchunk, Geo g, const DevState *st,
+    float *__restrict__ Ts, float *__restrict__ Tc, const float *__restrict__ partial,
+    DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int n = g.n, C = st->C;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+        const int nch = nchunk[c];
+        if (nch <= 1) continue;
+        const int w0 = wi_off[c];
+        const float cnt = (float)(cell_start[c + 1] - cell_start[c]);
+        for (int l = 0; l < D; l++) {
+            float s1 = 0.f, e1 = 0.f, s2 = 0.f, e2 = 0.f;
+            for (int k = 0; k < nch; k++) {
+                const float *pp = &partial[((long long)(w0 + k) * D + l) * 4];
+                two_sum_add(s1, e1, pp[0], pp[1]);
+                two_sum_add(s2, e2, pp[2], pp[3]);
+            }
+            const float a = (float)cell_g[(long long)l * n + c] * g.cs;
+            const float t1 = sinf(a) * cnt;
+            const float t2 = cosf(a) * cnt;
+            Ts[(long long)l * n + c] = t1 + (s1 + e1);
+            Tc[(long long)l * n + c] = t2 + (s2 + e2);
+        }
+    }
+}
+
+__global__ void k_store_total(const int *__restrict__ excl, const int *__restrict__ vals,
+                              DevState *st, DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int C = st->C;
+    st->W = C > 0 ? excl[C - 1] + vals[C - 1] : 0;
+}
+
+void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
+    const int n = g.n;
+    const int gb = grid_for(n);
+    k_wi_count<<<gb, kBlock, 0, s>>>(b.cell_start, st, b.nchunk, st_loop);
+    scan_exclusive(b.nchunk, b.wi_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
+    k_store_total<<<1, 1, 0, s>>>(b.wi_off, b.nchunk, st, st_loop);
+    // work items <= C + n / kSumChunk <= n + n / kSumChunk
+    const long long max_wi = (long long)n + n / kSumChunk + 1;
+    const int wb = grid_for(max_wi * kWave, kBlock, 16384);
+#define SUM_CASE(DD)                                                                         \
+    case DD:                                                                                 \
+        k_summary<DD><<<wb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.wi_off,          \
+                                            b.nchunk, g, st, b.Ts, b.Tc, b.partial, st_loop); \
+        k_summary_combine<DD><<<gb, kBlock, 0, s>>>(b.cell_start, b.cell_g, b.wi_off,        \
+                                                    b.nchunk, g, st, b.Ts, b.Tc, b.partial,  \
+                                                    st_loop);                                \
+        break;
+    switch (g.d) {
+        SUM_CASE(1) SUM_CASE(2) SUM_CASE(3) SUM_CASE(4)
+        SUM_CASE(5) SUM_CASE(6) SUM_CASE(7) SUM_CASE(8)
+    }
+#undef SUM_CASE
+}
+
+}  // namespace egg

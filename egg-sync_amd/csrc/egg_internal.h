@@ -1,0 +1,73 @@
+// Host-side launchers shared by the translation units of the library.
+#pragma once
+
+#include "egg_common.h"
+
+namespace egg {
+
+// Exclusive scan of n int32 values (n_dev, if given, overrides n at run time;
+// it must not exceed n).  `tmp` needs scan_tmp_ints(n) ints.  The sum of all
+// values is written to *total if total != nullptr.  In-place allowed.
+size_t scan_tmp_ints(int n);
+void scan_exclusive(const int *in, int *out, int n, const int *n_dev, int *tmp, int *total,
+                    const DevState *st, hipStream_t s);
+
+// Stable LSD radix sort of (key, position) pairs by key bits [0, key_bits),
+// 8 bits per pass.  keys_a/keys_b: [kw][n] u64; vals_a/vals_b: [n] int32.
+// On entry keys are in keys_a and vals are implicit (0..n-1).  Returns 0 if
+// the sorted result is in the *_a buffers, 1 if in *_b.
+size_t sort_tmp_ints(int n);
+int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int n, int kw,
+               int key_bits, int *hist, int *scan_tmp, const DevState *st, hipStream_t s);
+
+// Grid build, cell summaries (egg_grid.hip).
+struct GridBufs {
+    float *X;            // [d][n] current positions (state)
+    int *perm;           // [n]
+    float *Xs;           // [d][n] sorted positions
+    int *perm_s;         // [n]
+    uint64_t *keys_a, *keys_b;
+    int *vals_a, *vals_b;
+    int *hist;           // [256][ntiles]
+    int *scan_tmp;
+    int *head, *hpos;    // [n]
+    int *point_cell;     // [n]
+    int *cell_start;     // [n+1]
+    int *cell_g;         // [d][n]
+    int *cell_outer;     // [n]
+    int *ob, *oe;        // [O]
+    int *nchunk, *wi_off;// [n]
+    float *Ts, *Tc;      // [d][n]
+    float *partial;      // [n + n/kSumChunk + 1][4*d]
+};
+
+void launch_aos_to_soa(const float *aos, float *soa, int *perm, int n, int d, hipStream_t s);
+void launch_soa_to_aos_by_perm(const float *soa, const int *perm, float *aos, int n, int d,
+                               hipStream_t s);
+void launch_fill(int *p, int count, int v, const DevState *st, hipStream_t s);
+// keys + sort + gather + cells + outer ranges.  Writes st->C.
+void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s);
+void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s);
+
+// Update / extra check / r_c (egg_update.hip).
+struct DebugOut {
+    int *N, *half, *border, *synced;  // [n] per sorted position, nullable
+};
+void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *dbg,
+                       hipStream_t s);
+void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s);
+// SynC pass: positions (write_pos) and r_c in FP64 -> st->rc.
+void launch_update_sync(const Geo &g, GridBufs &b, DevState *st, double *rc_part, bool write_pos,
+                        hipStream_t s);
+int rc_partials(int n);
+void launch_set_flag(DevState *st, hipStream_t s);
+void launch_finish(DevState *st, hipStream_t s);
+
+// Labels (egg_labels.hip).
+void launch_labels_egg(const Geo &g, GridBufs &b, int *labels, int *multi, int *lab_off,
+                       DevState *st, hipStream_t s);
+void launch_labels_sync(const Geo &g, GridBufs &b, int *labels, int *invperm, DevState *st,
+                        hipStream_t s);
+void launch_validate(const float *aos, long long count, int *err, hipStream_t s);
+
+}  // namespace egg

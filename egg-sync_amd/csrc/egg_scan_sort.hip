@@ -1,0 +1,239 @@
+// Device-wide exclusive scan and stable LSD radix sort for gfx950.
+//
+// Scan: reduce-then-scan, 2048-element tiles (256 threads x 8 items), wave
+// scans with lane shuffles.  Radix sort: 8-bit digits, per-tile LDS histogram,
+// exclusive scan of the digit-major histogram, then a stable scatter in which
+// each wavefront ranks its 64 keys by digit with 8 ballots (lanes holding the
+// same digit form a 64-bit match mask).  Stability makes the grid build
+// deterministic: points that share a cell keep their previous order.
+#include "egg_internal.h"
+
+namespace egg {
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread over the block; also returns the total.
+__device__ __forceinline__ int block_excl_scan(int v, int *s_w, int &total) {
+    const int lane = __lane_id(), wid = threadIdx.x / kWave;
+    int inc = wave_incl_scan(v);
+    if (lane == kWave - 1) s_w[wid] = inc;
+    __syncthreads();
+    int woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; w++) {
+        int t = s_w[w];
+        if (w < wid) woff += t;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return woff + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int *__restrict__ in, int n,
+                                                        const int *n_dev, int *__restrict__ bsum,
+                                                        const DevState *st) {
+    if (st && st->done) return;
+    __shared__ int s_w[kBlock / kWave];
+    if (n_dev) n = min(n, *n_dev);
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) v += in[base + k];
+    int total;
+    block_excl_scan(v, s_w, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of nb block sums in chunks, total to *total.
+__global__ __launch_bounds__(kBlock) void k_scan_bsums(int *__restrict__ bsum, int nb,
+                                                       int *total, const DevState *st) {
+    if (st && st->done) return;
+    __shared__ int s_w[kBlock / kWave];
+    int carry = 0;
+    for (int base = 0; base < nb; base += kScanTile) {
+        int idx0 = base + threadIdx.x * kScanItems;
+        int vals[kScanItems];
+        int v = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            vals[k] = (idx0 + k < nb) ? bsum[idx0 + k] : 0;
+            v += vals[k];
+        }
+        int tot;
+        int ex = block_excl_scan(v, s_w, tot) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            if (idx0 + k < nb) bsum[idx0 + k] = ex;
+            ex += vals[k];
+        }
+        carry += tot;
+    }
+    if (total && threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_down(const int *in, int *out, int n,
+                                                      const int *n_dev,
+                                                      const int *__restrict__ bsum,
+                                                      const DevState *st) {
+    if (st && st->done) return;
+    __shared__ int s_w[kBlock / kWave];
+    if (n_dev) n = min(n, *n_dev);
+    const long long tile0 = (long long)blockIdx.x * kScanTile;
+    if (tile0 >= n) return;
+    const long long base = tile0 + (long long)threadIdx.x * kScanItems;
+    int vals[kScanItems];
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        vals[k] = (base + k < n) ? in[base + k] : 0;
+        v += vals[k];
+    }
+    int tot;
+    int ex = block_excl_scan(v, s_w, tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = ex;
+        ex += vals[k];
+    }
+}
+
+size_t scan_tmp_ints(int n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+void scan_exclusive(const int *in, int *out, int n, const int *n_dev, int *tmp, int *total,
+                    const DevState *st, hipStream_t s) {
+    int nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) nb = 1;
+    k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, n_dev, tmp, st);
+    k_scan_bsums<<<1, kBlock, 0, s>>>(tmp, nb, total, st);
+    k_scan_down<<<nb, kBlock, 0, s>>>(in, out, n, n_dev, tmp, st);
+}
+
+// ------------------------------------------------------------ radix sort --
+
+__device__ __forceinline__ int digit_of(const uint64_t *keys, int n, int e, int shift) {
+    const uint64_t w = keys[(long long)(shift >> 6) * n + e];
+    return (int)((w >> (shift & 63)) & 255ull);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rs_hist(const uint64_t *__restrict__ keys, int n,
+                                                    int shift, int *__restrict__ hist,
+                                                    int ntiles, const DevState *st) {
+    if (st && st->done) return;
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+        long long e = base + k * kBlock + threadIdx.x;
+        if (e < n) atomicAdd(&h[digit_of(keys, n, (int)e, shift)], 1);
+    }
+    __syncthreads();
+    hist[(long long)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+template <int KW>
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint64_t *__restrict__ kin,
+                                                       const int *__restrict__ vin,
+                                                       uint64_t *__restrict__ kout,
+                                                       int *__restrict__ vout, int n, int shift,
+                                                       const int *__restrict__ hist_scan,
+                                                       int ntiles, const DevState *st) {
+    if (st && st->done) return;
+    __shared__ int s_off[256];
+    __shared__ int s_wc[kBlock / kWave][256];
+    const int t = threadIdx.x, wid = t / kWave;
+    s_off[t] = hist_scan[(long long)t * ntiles + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; w++) s_wc[w][t] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kSortTile;
+    const int word = shift >> 6, sh = shift & 63;
+    for (int k = 0; k < kSortItems; k++) {
+        const long long e = base + k * kBlock + t;
+        const bool valid = e < n;
+        uint64_t kv[KW];
+        int val = 0, dig = 0;
+        if (valid) {
+#pragma unroll
+            for (int q = 0; q < KW; q++) kv[q] = kin[(long long)q * n + e];
+            val = vin ? vin[e] : (int)e;
+            dig = (int)((kv[word] >> sh) & 255ull);
+        }
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (dig >> b) & 1;
+            const unsigned long long bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const int r = __popcll(m & lanemask_lt());
+        if (valid && r == 0) s_wc[wid][dig] = __popcll(m);
+        __syncthreads();
+        long long pos = 0;
+        if (valid) {
+            pos = s_off[dig] + r;
+            for (int w = 0; w < wid; w++) pos += s_wc[w][dig];
+        }
+        __syncthreads();
+        int tot = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; w++) {
+            tot += s_wc[w][t];
+            s_wc[w][t] = 0;
+        }
+        s_off[t] += tot;
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int q = 0; q < KW; q++) kout[(long long)q * n + pos] = kv[q];
+            vout[pos] = val;
+        }
+    }
+}
+
+size_t sort_tmp_ints(int n) {
+    int ntiles = (n + kSortTile - 1) / kSortTile;
+    if (ntiles == 0) ntiles = 1;
+    return (size_t)256 * ntiles;
+}
+
+int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int n, int kw,
+               int key_bits, int *hist, int *scan_tmp, const DevState *st, hipStream_t s) {
+    int ntiles = (n + kSortTile - 1) / kSortTile;
+    if (ntiles == 0) ntiles = 1;
+    const int passes = (key_bits + 7) / 8;
+    uint64_t *kin = keys_a, *kout = keys_b;
+    int *vin = nullptr, *vout = vals_b;
+    int where = 0;
+    for (int p = 0; p < passes; p++) {
+        const int shift = p * 8;
+        k_rs_hist<<<ntiles, kBlock, 0, s>>>(kin, n, shift, hist, ntiles, st);
+        scan_exclusive(hist, hist, 256 * ntiles, nullptr, scan_tmp, nullptr, st, s);
+        if (kw == 1)
+            k_rs_scatter<1><<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, hist, ntiles, st);
+        else
+            k_rs_scatter<2><<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, hist, ntiles, st);
+        where ^= 1;
+        uint64_t *tk = kin; kin = kout; kout = tk;
+        vin = vout;
+        vout = (vout == vals_b) ? vals_a : vals_b;
+    }
+    if (passes == 0) {
+        // a single cell: identity order (keys unchanged in keys_a)
+        return -1;
+    }
+    return where;
+}
+
+}  // namespace egg

@@ -1,0 +1,634 @@
+// Host driver and C ABI of the EGG-SynC solver (EGG_SynC.cu:1229-1601).
+//
+// Differences in structure from the reference driver, same algorithm:
+//   - one cached device workspace per device instead of ~25 cudaMallocs per
+//     call and per-iteration mallocs in scans and the extra check;
+//   - the loop state lives on the device (DevState): every kernel of an
+//     iteration returns at once when `done` is set, so the host enqueues
+//     `iters_per_sync` iterations and reads the state back once per batch
+//     instead of 2-4 blocking copies per iteration (EGG_SynC.cu:1368,1416,1449,1483);
+//   - errors come back as codes, never exit().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/egg_sync.h"
+#include "egg_internal.h"
+
+namespace egg {
+namespace {
+
+std::mutex g_mutex;
+
+constexpr int kTraceCap = 4096;
+
+struct Workspace {
+    void *base = nullptr;
+    size_t bytes = 0;
+    hipStream_t stream = nullptr;
+    DevState *h_state = nullptr;  // pinned readback
+};
+
+Workspace g_ws[64];
+
+size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+int bitlen(long long v) {
+    int b = 0;
+    while (v > 0) {
+        b++;
+        v >>= 1;
+    }
+    return b < 1 ? 1 : b;
+}
+
+float sqrt_threshold(float e) {
+    float s = e * e;
+    if (std::isinf(s)) s = 3.4028234663852886e38f;
+    while (s > 0.0f && std::sqrt(s) > e) s = std::nextafter(s, 0.0f);
+    for (;;) {
+        float nx = std::nextafter(s, INFINITY);
+        if (std::isinf(nx) || std::sqrt(nx) > e) break;
+        s = nx;
+    }
+    return s;
+}
+
+// Geometry exactly as EGG_SynC.cu:1249-1260, :563, :779, :1045, :1430.
+int compute_geo(int64_t n, int d, float eps, Geo &g) {
+    if (n < 1 || n > (int64_t)(INT_MAX / 2)) return EGG_SYNC_E_ARG;
+    if (d < 1) return EGG_SYNC_E_ARG;
+    if (d > kMaxD) return EGG_SYNC_E_DIM;
+    if (!(eps > 0.0f) || !std::isfinite(eps)) return EGG_SYNC_E_ARG;
+    std::memset(&g, 0, sizeof g);
+    g.n = (int)n;
+    g.d = d;
+    g.eps = eps;
+    g.cs = (float)std::sqrt(std::pow((double)eps / 2., 2.) / d);
+    const float inv = 1 / g.cs;
+    int w = inv >= 2147483647.0f ? 2147483647 : (int)inv;
+    if (w < 1) w = 1;
+    g.w = w;
+    int od;
+    if (w == 1 || n <= 1) {
+        od = 1;
+    } else {
+        od = (int)(std::log((double)n) / std::log((double)w));
+        od = std::max(1, std::min(d, od));
+    }
+    g.od = od;
+    long long O = 1;
+    for (int l = 0; l < od; l++) {
+        g.wpow[l] = (int)O;
+        O *= w;
+        if (O > INT_MAX) return EGG_SYNC_E_KEY;
+    }
+    g.O = (int)O;
+    g.R = (int)std::ceil(eps / g.cs);
+    const float e2 = 2 * eps;
+    const double ex = (double)e2 - (double)eps * std::sqrt(15. / 16.) + (double)eps / 2. -
+                      std::sin((double)eps / 2.);
+    g.eps_extra = (float)ex;
+    g.R_extra = (int)std::ceil(g.eps_extra / g.cs);
+    g.R_half = (int)std::ceil(((double)eps / 2.) / (double)g.cs);
+    g.thr_eps = sqrt_threshold(eps);
+    g.thr_half = sqrt_threshold(eps / 2);
+    g.thr_extra = sqrt_threshold(g.eps_extra);
+    g.bits_w = bitlen((long long)w - 1);
+    g.bits_o = bitlen(O - 1);
+    g.key_bits = g.bits_o + (d - od) * g.bits_w;
+    if (g.key_bits > 128) return EGG_SYNC_E_KEY;
+    g.kw = g.key_bits > 64 ? 2 : 1;
+    return EGG_SYNC_OK;
+}
+
+struct Carve {
+    char *p;
+    size_t off = 0;
+    explicit Carve(void *base) : p((char *)base) {}
+    template <typename T>
+    T *take(size_t count) {
+        T *r = p ? (T *)(p + off) : nullptr;
+        off += align_up(sizeof(T) * std::max<size_t>(count, 1));
+        return r;
+    }
+};
+
+struct Layout {
+    GridBufs b{};
+    DevState *st = nullptr;
+    int *labels = nullptr;    // [n]
+    float *io = nullptr;      // [n*d] input / output staging
+    double *rc_part = nullptr;
+    double *rc_trace = nullptr;
+    int *err = nullptr;
+    size_t bytes = 0;
+};
+
+Layout carve(void *base, const Geo &g) {
+    Layout L;
+    Carve c(base);
+    const size_t n = (size_t)g.n, d = (size_t)g.d;
+    GridBufs &b = L.b;
+    b.X = c.take<float>(n * d);
+    b.Xs = c.take<float>(n * d);
+    b.perm = c.take<int>(n);
+    b.perm_s = c.take<int>(n);
+    b.keys_a = c.take<uint64_t>(n * g.kw);
+    b.keys_b = c.take<uint64_t>(n * g.kw);
+    b.vals_a = c.take<int>(n);
+    b.vals_b = c.take<int>(n);
+    const size_t hist = sort_tmp_ints(g.n);
+    b.hist = c.take<int>(hist);
+    b.scan_tmp = c.take<int>(scan_tmp_ints((int)std::max(n, hist)) + 16);
+    b.head = c.take<int>(n);
+    b.hpos = c.take<int>(n);
+    b.point_cell = c.take<int>(n);
+    b.cell_start = c.take<int>(n + 1);
+    b.cell_g = c.take<int>(n * d);
+    b.cell_outer = c.take<int>(n);
+    b.ob = c.take<int>((size_t)g.O);
+    b.oe = c.take<int>((size_t)g.O);
+    b.nchunk = c.take<int>(n);
+    b.wi_off = c.take<int>(n);
+    b.Ts = c.take<float>(n * d);
+    b.Tc = c.take<float>(n * d);
+    b.partial = c.take<float>((n + n / kSumChunk + 1) * 4 * d);
+    L.st = c.take<DevState>(1);
+    L.labels = c.take<int>(n);
+    L.io = c.take<float>(n * d);
+    L.rc_part = c.take<double>((size_t)rc_partials(g.n));
+    L.rc_trace = c.take<double>(kTraceCap);
+    L.err = c.take<int>(1);
+    L.bytes = c.off;
+    return L;
+}
+
+int hip_err(hipError_t e) {
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorOutOfMemory) return EGG_SYNC_E_ALLOC;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return EGG_SYNC_E_NODEV;
+    return EGG_SYNC_E_HIP;
+}
+
+#define HIP_TRY(x)                       \
+    do {                                 \
+        int rc_ = hip_err((x));          \
+        if (rc_) return rc_;             \
+    } while (0)
+
+int acquire(const Geo &g, Layout &L, Workspace *&ws) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return EGG_SYNC_E_NODEV;
+    ws = &g_ws[dev];
+    Layout probe = carve(nullptr, g);
+    if (ws->bytes < probe.bytes) {
+        if (ws->base) HIP_TRY(hipFree(ws->base));
+        ws->base = nullptr;
+        ws->bytes = 0;
+        size_t want = probe.bytes + probe.bytes / 8;
+        if (hipMalloc(&ws->base, want) != hipSuccess) {
+            (void)hipGetLastError();
+            ws->base = nullptr;
+            return EGG_SYNC_E_ALLOC;
+        }
+        ws->bytes = want;
+    }
+    if (!ws->stream) HIP_TRY(hipStreamCreate(&ws->stream));
+    if (!ws->h_state) HIP_TRY(hipHostMalloc((void **)&ws->h_state, sizeof(DevState), 0));
+    L = carve(ws->base, g);
+    return 0;
+}
+
+__global__ void k_aos_to_soa_order(const float *__restrict__ aos, const int *__restrict__ order,
+                                   float *__restrict__ soa, int *__restrict__ perm, int n, int d) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int p = order ? order[i] : (int)i;
+        for (int l = 0; l < d; l++) soa[(long long)l * n + i] = aos[(long long)p * d + l];
+        perm[i] = p;
+    }
+}
+
+__global__ void k_soa_to_rows(const int *__restrict__ soa, int n, int d, const DevState *st,
+                              int *__restrict__ rows) {
+    const int C = st->C;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < C;
+         c += (long long)gridDim.x * blockDim.x)
+        for (int l = 0; l < d; l++) rows[c * d + l] = soa[(long long)l * n + c];
+}
+
+__global__ void k_snapshot_flag(DevState *st) {
+    if (st->done) return;
+    st->flag_local = st->flag;
+}
+
+__global__ void k_trace_rc(DevState *st, double *trace, int cap) {
+    // called after k_finish: iteration `iter` (1-based) just completed
+    const int it = st->iter;
+    if (it >= 1 && it <= cap) trace[it - 1] = st->rc;
+}
+
+int grid_blocks(long long n) {
+    long long b = (n + kBlock - 1) / kBlock;
+    return (int)std::max(1LL, std::min(b, 65536LL));
+}
+
+void init_state(DevState &h, const egg_sync_options &opt) {
+    std::memset(&h, 0, sizeof h);
+    h.stop_rule = opt.stop_rule;
+    h.lam = opt.lam;
+    h.max_iter = opt.max_iter;
+    h.flag = 1;
+    h.rc = std::nan("");
+}
+
+void default_options(egg_sync_options &o) {
+    std::memset(&o, 0, sizeof o);
+    o.stop_rule = EGG_SYNC_STOP_EGG;
+    o.label_rule = EGG_SYNC_LABELS_CELLS;
+    o.lam = 1.0f - 1e-3f;
+    o.max_iter = 0;
+    o.compute_rc = 0;
+    o.timing = 0;
+    o.iters_per_sync = 4;
+    o.validate = 1;
+}
+
+struct Events {
+    std::vector<hipEvent_t> ev;
+    int per_iter = 4;
+    int ensure(int count) {
+        while ((int)ev.size() < count) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        return 0;
+    }
+    ~Events() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+double ms_between(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0;
+    }
+    return (double)ms;
+}
+
+// One pass of the loop body (EGG_SynC.cu:1323-1491), fully on the device.
+void enqueue_iteration(const Geo &g, Layout &L, const egg_sync_options &opt, hipStream_t s,
+                       hipEvent_t *ev) {
+    if (ev) (void)hipEventRecord(ev[0], s);
+    launch_set_flag(L.st, s);
+    build_grid(g, L.b, L.st, L.st, s);
+    if (opt.stop_rule == EGG_SYNC_STOP_EGG) cell_summaries(g, L.b, L.st, L.st, s);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    if (opt.stop_rule == EGG_SYNC_STOP_EGG) {
+        launch_update_egg(g, L.b, L.st, nullptr, s);
+        if (ev) (void)hipEventRecord(ev[2], s);
+        k_snapshot_flag<<<1, 1, 0, s>>>(L.st);
+        launch_extra_check(g, L.b, L.st, s);
+    } else {
+        launch_update_sync(g, L.b, L.st, L.rc_part, true, s);
+        if (ev) (void)hipEventRecord(ev[2], s);
+    }
+    launch_finish(L.st, s);
+    k_trace_rc<<<1, 1, 0, s>>>(L.st, L.rc_trace, kTraceCap);
+    if (ev) (void)hipEventRecord(ev[3], s);
+}
+
+int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
+                      const egg_sync_options &opt, int32_t *d_labels, float *d_final,
+                      egg_sync_stats *so, hipStream_t s_user, const float *h_D,
+                      int32_t *h_labels, float *h_final) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    if (opt.stop_rule != EGG_SYNC_STOP_EGG && opt.stop_rule != EGG_SYNC_STOP_RC)
+        return EGG_SYNC_E_ARG;
+    if (opt.label_rule != EGG_SYNC_LABELS_CELLS && opt.label_rule != EGG_SYNC_LABELS_SYNC)
+        return EGG_SYNC_E_ARG;
+    if (n64 < 0 || d < 1) return EGG_SYNC_E_ARG;
+    if (so) {
+        so->iterations = 0;
+        so->status = 0;
+        so->n_clusters = 0;
+        so->n_cells = 0;
+        so->rc = std::nan("");
+        for (int k = 0; k < 6; k++) so->stage_ms[k] = 0.0;
+        so->update_kernel_ms = 0.0;
+        so->update_launches = 0;
+        so->workspace_bytes = 0;
+    }
+    if (n64 == 0) {
+        if (so) so->total_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        return EGG_SYNC_OK;
+    }
+    Geo g;
+    int rc = compute_geo(n64, d, eps, g);
+    if (rc) return rc;
+    const int n = g.n;
+    Layout L;
+    Workspace *ws = nullptr;
+    rc = acquire(g, L, ws);
+    if (rc) return rc;
+    hipStream_t s = s_user ? s_user : ws->stream;
+    if (h_D) {
+        HIP_TRY(hipMemcpyAsync(L.io, h_D, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice, s));
+        d_D = L.io;
+    }
+    if (opt.validate) {
+        HIP_TRY(hipMemsetAsync(L.err, 0, sizeof(int), s));
+        launch_validate(d_D, (long long)n * d, L.err, s);
+        int herr = 0;
+        HIP_TRY(hipMemcpyAsync(&herr, L.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (herr) return EGG_SYNC_E_RANGE;
+    }
+    launch_aos_to_soa(d_D, L.b.X, L.b.perm, n, d, s);
+    DevState hs;
+    init_state(hs, opt);
+    HIP_TRY(hipMemcpyAsync(L.st, &hs, sizeof hs, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const auto t_alloc = clk::now();
+
+    const int K = std::max(1, opt.iters_per_sync);
+    Events E;
+    if (opt.timing) {
+        rc = E.ensure(4 * K + 4);
+        if (rc) return rc;
+    }
+    double st_build = 0, st_upd = 0, st_extra = 0;
+    int done_iters = 0;
+    for (;;) {
+        for (int k = 0; k < K; k++)
+            enqueue_iteration(g, L, opt, s, opt.timing ? &E.ev[4 * k] : nullptr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(ws->h_state, L.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const int real = ws->h_state->iter - done_iters;
+        if (opt.timing) {
+            for (int k = 0; k < real && k < K; k++) {
+                hipEvent_t *ev = &E.ev[4 * k];
+                const double b = ms_between(ev[0], ev[1]);
+                const double u = ms_between(ev[1], ev[2]);
+                const double x = ms_between(ev[2], ev[3]);
+                st_build += b;
+                st_upd += u;
+                st_extra += x;
+                if (so && so->itr_ms && done_iters + k < so->itr_ms_cap)
+                    so->itr_ms[done_iters + k] = (float)(b + u + x);
+            }
+        }
+        done_iters = ws->h_state->iter;
+        if (ws->h_state->done) break;
+    }
+    const DevState loop_state = *ws->h_state;
+
+    // final grid on the converged positions and labels (EGG_SynC.cu:1504-1541)
+    hipEvent_t *fe = opt.timing ? &E.ev[4 * K] : nullptr;
+    if (fe) (void)hipEventRecord(fe[0], s);
+    build_grid(g, L.b, nullptr, L.st, s);
+    int *labels_dev = d_labels ? d_labels : L.labels;
+    if (opt.label_rule == EGG_SYNC_LABELS_CELLS)
+        launch_labels_egg(g, L.b, labels_dev, L.b.nchunk, L.b.wi_off, L.st, s);
+    else
+        launch_labels_sync(g, L.b, labels_dev, L.b.head, L.st, s);
+    if (opt.compute_rc && opt.stop_rule == EGG_SYNC_STOP_EGG)
+        launch_update_sync(g, L.b, L.st, L.rc_part, false, s);
+    if (fe) (void)hipEventRecord(fe[1], s);
+    float *final_dev = d_final ? d_final : (h_final ? L.io : nullptr);
+    if (final_dev) launch_soa_to_aos_by_perm(L.b.Xs, L.b.perm_s, final_dev, n, d, s);
+    if (h_labels)
+        HIP_TRY(hipMemcpyAsync(h_labels, labels_dev, sizeof(int) * (size_t)n,
+                               hipMemcpyDeviceToHost, s));
+    if (h_final)
+        HIP_TRY(hipMemcpyAsync(h_final, final_dev, sizeof(float) * (size_t)n * d,
+                               hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ws->h_state, L.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+    if (fe) (void)hipEventRecord(fe[2], s);
+    HIP_TRY(hipStreamSynchronize(s));
+    const DevState fin = *ws->h_state;
+    if (so) {
+        so->iterations = loop_state.iter;
+        so->status = loop_state.capped ? EGG_SYNC_CAPPED : EGG_SYNC_OK;
+        so->n_clusters = fin.n_clusters;
+        so->n_cells = fin.C;
+        so->rc = (opt.compute_rc || opt.stop_rule == EGG_SYNC_STOP_RC) ? fin.rc : std::nan("");
+        so->stage_ms[0] = std::chrono::duration<double, std::milli>(t_alloc - t0).count();
+        so->stage_ms[1] = st_build;
+        so->stage_ms[2] = st_upd;
+        so->stage_ms[3] = st_extra;
+        if (fe) {
+            so->stage_ms[4] = ms_between(fe[0], fe[1]);
+            so->stage_ms[5] = ms_between(fe[1], fe[2]);
+        }
+        so->update_kernel_ms = st_upd;
+        so->update_launches = opt.timing ? loop_state.iter : 0;
+        so->workspace_bytes = (int64_t)L.bytes;
+        so->total_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    }
+    return loop_state.capped ? EGG_SYNC_CAPPED : EGG_SYNC_OK;
+}
+
+// Shared set-up of the single-pass entry points: positions (point-id order)
+// visited in `order` into X/perm, loop state reset.
+int setup_single(const float *d_D, int64_t n64, int d, float eps, const int32_t *d_order,
+                 Geo &g, Layout &L, hipStream_t &s, void *stream) {
+    if (!d_D) return EGG_SYNC_E_ARG;
+    int rc = compute_geo(n64, d, eps, g);
+    if (rc) return rc;
+    Workspace *ws = nullptr;
+    rc = acquire(g, L, ws);
+    if (rc) return rc;
+    s = stream ? (hipStream_t)stream : ws->stream;
+    k_aos_to_soa_order<<<grid_blocks(g.n), kBlock, 0, s>>>(d_D, d_order, L.b.X, L.b.perm, g.n, d);
+    egg_sync_options opt;
+    default_options(opt);
+    DevState hs;
+    init_state(hs, opt);
+    HIP_TRY(hipMemcpyAsync(L.st, &hs, sizeof hs, hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+}  // namespace
+}  // namespace egg
+
+using namespace egg;
+
+extern "C" {
+
+int egg_sync_default_options(egg_sync_options *opt) {
+    if (!opt) return EGG_SYNC_E_ARG;
+    default_options(*opt);
+    return EGG_SYNC_OK;
+}
+
+int egg_sync_solve(const float *h_D, int64_t n, int32_t d, float eps, int32_t stop_rule,
+                   int32_t *labels_out, egg_sync_stats *stats_out) {
+    egg_sync_options opt;
+    default_options(opt);
+    opt.stop_rule = stop_rule;
+    if (stop_rule == EGG_SYNC_STOP_RC) opt.label_rule = EGG_SYNC_LABELS_SYNC;
+    return egg_sync_solve_ex(h_D, n, d, eps, &opt, labels_out, nullptr, stats_out);
+}
+
+int egg_sync_solve_ex(const float *h_D, int64_t n, int32_t d, float eps,
+                      const egg_sync_options *opt, int32_t *labels_out, float *final_out,
+                      egg_sync_stats *stats_out) {
+    if ((!h_D || !labels_out) && n > 0) return EGG_SYNC_E_ARG;
+    egg_sync_options o;
+    if (opt) o = *opt;
+    else default_options(o);
+    std::lock_guard<std::mutex> lk(g_mutex);
+    return solve_device_impl(nullptr, n, d, eps, o, nullptr, nullptr, stats_out, nullptr, h_D,
+                             labels_out, final_out);
+}
+
+int egg_sync_solve_device(const float *d_D, int64_t n, int32_t d, float eps,
+                          const egg_sync_options *opt, int32_t *d_labels, float *d_final,
+                          egg_sync_stats *stats_out, void *stream) {
+    if ((!d_D || !d_labels) && n > 0) return EGG_SYNC_E_ARG;
+    egg_sync_options o;
+    if (opt) o = *opt;
+    else default_options(o);
+    std::lock_guard<std::mutex> lk(g_mutex);
+    return solve_device_impl(d_D, n, d, eps, o, d_labels, d_final, stats_out,
+                             (hipStream_t)stream, nullptr, nullptr, nullptr);
+}
+
+int egg_sync_build_grid_device(const float *d_D, int64_t n, int32_t d, float eps,
+                               const int32_t *d_order_in, int32_t *d_order_out,
+                               int32_t *d_cell_of, int32_t *d_cell_start, int32_t *d_cell_g,
+                               int32_t *d_ob, int32_t *d_oe, int32_t *C_out, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    Geo g;
+    Layout L;
+    hipStream_t s;
+    int rc = setup_single(d_D, n, d, eps, d_order_in, g, L, s, stream);
+    if (rc) return rc;
+    build_grid(g, L.b, nullptr, L.st, s);
+    const size_t nn = (size_t)g.n;
+    if (d_order_out)
+        HIP_TRY(hipMemcpyAsync(d_order_out, L.b.perm_s, 4 * nn, hipMemcpyDeviceToDevice, s));
+    if (d_cell_of)
+        HIP_TRY(hipMemcpyAsync(d_cell_of, L.b.point_cell, 4 * nn, hipMemcpyDeviceToDevice, s));
+    if (d_cell_g) k_soa_to_rows<<<grid_blocks(g.n), kBlock, 0, s>>>(L.b.cell_g, g.n, d, L.st, d_cell_g);
+    if (d_ob) HIP_TRY(hipMemcpyAsync(d_ob, L.b.ob, 4 * (size_t)g.O, hipMemcpyDeviceToDevice, s));
+    if (d_oe) HIP_TRY(hipMemcpyAsync(d_oe, L.b.oe, 4 * (size_t)g.O, hipMemcpyDeviceToDevice, s));
+    DevState hs;
+    HIP_TRY(hipMemcpyAsync(&hs, L.st, sizeof hs, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (d_cell_start)
+        HIP_TRY(hipMemcpy(d_cell_start, L.b.cell_start, 4 * ((size_t)hs.C + 1),
+                          hipMemcpyDeviceToDevice));
+    if (C_out) *C_out = hs.C;
+    HIP_TRY(hipGetLastError());
+    return EGG_SYNC_OK;
+}
+
+int egg_sync_iteration_device(const float *d_D, int64_t n, int32_t d, float eps,
+                              const int32_t *d_order_in, float *d_next, int32_t *d_order_out,
+                              int32_t *d_N, int32_t *d_half, int32_t *d_border,
+                              int32_t *d_synced, int32_t *flag_local, int32_t *flag_final,
+                              void *stream) {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    Geo g;
+    Layout L;
+    hipStream_t s;
+    int rc = setup_single(d_D, n, d, eps, d_order_in, g, L, s, stream);
+    if (rc) return rc;
+    DebugOut dbg{d_N, d_half, d_border, d_synced};
+    launch_set_flag(L.st, s);
+    build_grid(g, L.b, L.st, L.st, s);
+    cell_summaries(g, L.b, L.st, L.st, s);
+    launch_update_egg(g, L.b, L.st, &dbg, s);
+    k_snapshot_flag<<<1, 1, 0, s>>>(L.st);
+    launch_extra_check(g, L.b, L.st, s);
+    if (d_next) launch_soa_to_aos_by_perm(L.b.X, L.b.perm, d_next, g.n, d, s);
+    if (d_order_out)
+        HIP_TRY(hipMemcpyAsync(d_order_out, L.b.perm, 4 * (size_t)g.n, hipMemcpyDeviceToDevice, s));
+    DevState hs;
+    HIP_TRY(hipMemcpyAsync(&hs, L.st, sizeof hs, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipGetLastError());
+    if (flag_local) *flag_local = hs.flag_local;
+    if (flag_final) *flag_final = hs.flag;
+    return EGG_SYNC_OK;
+}
+
+int egg_sync_rc_device(const float *d_D, int64_t n, int32_t d, float eps, double *rc_out,
+                       void *stream) {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    Geo g;
+    Layout L;
+    hipStream_t s;
+    int rc = setup_single(d_D, n, d, eps, nullptr, g, L, s, stream);
+    if (rc) return rc;
+    build_grid(g, L.b, nullptr, L.st, s);
+    launch_update_sync(g, L.b, L.st, L.rc_part, false, s);
+    DevState hs;
+    HIP_TRY(hipMemcpyAsync(&hs, L.st, sizeof hs, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipGetLastError());
+    if (rc_out) *rc_out = hs.rc;
+    return EGG_SYNC_OK;
+}
+
+int egg_sync_geometry(int64_t n, int32_t d, float eps, int64_t *out5, float *cs_out) {
+    Geo g;
+    int rc = compute_geo(n, d, eps, g);
+    if (rc) return rc;
+    if (out5) {
+        out5[0] = g.w;
+        out5[1] = g.od;
+        out5[2] = g.O;
+        out5[3] = g.R;
+        out5[4] = g.key_bits;
+    }
+    if (cs_out) *cs_out = g.cs;
+    return EGG_SYNC_OK;
+}
+
+int egg_sync_release(void) {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return EGG_SYNC_E_NODEV;
+    if (dev < 0 || dev >= 64) return EGG_SYNC_E_NODEV;
+    Workspace &ws = g_ws[dev];
+    if (ws.base) (void)hipFree(ws.base);
+    if (ws.stream) (void)hipStreamDestroy(ws.stream);
+    if (ws.h_state) (void)hipHostFree(ws.h_state);
+    ws = Workspace{};
+    return EGG_SYNC_OK;
+}
+
+const char *egg_sync_error_string(int code) {
+    switch (code) {
+        case EGG_SYNC_OK: return "ok";
+        case EGG_SYNC_CAPPED: return "iteration cap reached before convergence";
+        case EGG_SYNC_E_ARG: return "invalid argument";
+        case EGG_SYNC_E_RANGE: return "input values must be finite and lie in [0, 1]";
+        case EGG_SYNC_E_DIM: return "dimension above EGG_SYNC_MAX_D";
+        case EGG_SYNC_E_KEY: return "grid too fine: cell keys exceed 128 bits";
+        case EGG_SYNC_E_ALLOC: return "device allocation failed";
+        case EGG_SYNC_E_HIP: return "HIP runtime error";
+        case EGG_SYNC_E_NODEV: return "no GPU device";
+        default: return "unknown error";
+    }
+}
+
+}  // extern "C"

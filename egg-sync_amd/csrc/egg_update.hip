@@ -1,0 +1,519 @@
+// Neighbour update, local-sync flag, extra check and FP64 order parameter
+// (EGG_SynC.cu:551-763, 767-1203; SynC.cpp:30-56), gfx950.
+//
+// One thread per point in cell-sorted order, so the lanes of a wavefront
+// mostly share one cell and walk the same candidate cells: cell metadata and
+// neighbour coordinates are wave-uniform loads served by the cache.  Candidate
+// cells come from the outer-cell box of radius R (as the reference), pruned
+// in the remaining dimensions to |dg| <= R+1 by a binary search over the
+// column's cells (sorted by g_od) -- cells farther than that fail the
+// reference's box test by more than a cell side, so results do not change.
+#include "egg_internal.h"
+
+namespace egg {
+
+// Odometer over the outer-cell offsets [-R, R]^od, dim 0 fastest (the
+// enumeration order of EGG_SynC.cu:407-451).  Unrolled over D so the offsets
+// stay in registers.
+template <int D>
+struct Odo {
+    int idx[D];
+    bool finished;
+    __device__ __forceinline__ void init(int od, int R) {
+#pragma unroll
+        for (int l = 0; l < D; l++) idx[l] = (l < od) ? -R : 0;
+        finished = false;
+    }
+    __device__ __forceinline__ void next(int od, int R) {
+        bool carry = true;
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            if (l < od && carry) {
+                idx[l]++;
+                if (idx[l] > R) {
+                    if (l == od - 1) finished = true;
+                    else idx[l] = -R;
+                } else {
+                    carry = false;
+                }
+            }
+        }
+    }
+    // outer cell id of the current offset, or -1 if outside the grid
+    __device__ __forceinline__ int outer(const int *g0, const Geo &g) const {
+        int o = 0;
+        bool ok = true;
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            if (l < g.od) {
+                const int v = g0[l] + idx[l];
+                if (v < 0 || v >= g.w) ok = false;
+                o += v * g.wpow[l];
+            }
+        }
+        return ok ? o : -1;
+    }
+};
+
+// Cells of outer cell o whose first remaining coordinate lies within
+// [g0od - rad, g0od + rad]: returns the first such cell (cells of a column are
+// sorted by g_od); the caller stops at the first cell above the range.
+__device__ __forceinline__ int column_lower(const int *__restrict__ cell_g_od, int cb, int ce,
+                                            int lo_v) {
+    int lo = cb, hi = ce;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cell_g_od[mid] < lo_v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int D>
+__device__ __forceinline__ int select_dim(const int *v, int k) {
+    int r = 0;
+#pragma unroll
+    for (int l = 0; l < D; l++)
+        if (l == k) r = v[l];
+    return r;
+}
+
+template <int D>
+__device__ __forceinline__ float dist2_to(const float *x, const float *__restrict__ Xs, int n,
+                                          int j, float *xq) {
+    float s = 0.f;
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        xq[l] = Xs[(long long)l * n + j];
+        const float df = x[l] - xq[l];
+        s += df * df;
+    }
+    return s;
+}
+
+// kernel_itr_grid_STAD5_5_1 (EGG_SynC.cu:551-763).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_update_egg(
+    const float *__restrict__ Xs, float *__restrict__ Xout, const int *__restrict__ perm_s,
+    int *__restrict__ perm_out, const int *__restrict__ point_cell,
+    const int *__restrict__ cell_start, const int *__restrict__ cell_g,
+    const float *__restrict__ Ts, const float *__restrict__ Tc, const int *__restrict__ ob,
+    const int *__restrict__ oe, Geo g, DevState *st, DebugOut dbg) {
+    if (st->done) return;
+    const int n = g.n;
+    const int R = g.R, od = g.od;
+    const float thr = g.thr_eps, thr_half = g.thr_half;
+    bool unsynced = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float x[D], sx[D], cx[D], sum[D], sml[D];
+        int g0[D];
+        const int c0 = point_cell[i];
+        const int own = cell_start[c0 + 1] - cell_start[c0];
+        bool border = false;
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            x[l] = Xs[(long long)l * n + i];
+            g0[l] = cell_g[(long long)l * n + c0];
+            sum[l] = 0.f;
+            sml[l] = 0.f;
+            sx[l] = sinf(x[l]);
+            cx[l] = cosf(x[l]);
+            border |= on_border(x[l], g.cs);
+        }
+        const int g0od = select_dim<D>(g0, od);
+        const int *cg_od = cell_g + (long long)od * n;
+        int N = 0, half = 0;
+        Odo<D> odo;
+        odo.init(od, R);
+        while (!odo.finished) {
+            const int o = odo.outer(g0, g);
+            if (o >= 0) {
+                int cb = ob[o];
+                const int ce = oe[o];
+                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+                for (int c = cb; c < ce; c++) {
+                    int gc[D];
+#pragma unroll
+                    for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
+                    if (od < D && select_dim<D>(gc, od) > g0od + R + 1) break;
+                    bool skip = false;
+#pragma unroll
+                    for (int l = 0; l < D; l++)
+                        if (l > od && (gc[l] - g0[l] > R + 1 || g0[l] - gc[l] > R + 1)) skip = true;
+                    if (skip) continue;
+                    float sm, lg;
+                    box_dist<D>(gc, x, g.cs, sm, lg);
+                    const bool included = sm <= thr;
+                    const bool full = lg <= thr;
+                    const int s0 = cell_start[c], s1 = cell_start[c + 1];
+                    if (full) {
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            const float a = Ts[(long long)l * n + c] * cx[l];
+                            const float b = Tc[(long long)l * n + c] * sx[l];
+                            const float t = a - b;
+                            sum[l] += t;
+                        }
+                        N += s1 - s0;
+                        if (border) {
+                            if (c == c0) {
+                                half += s1 - s0;
+                            } else {
+                                for (int j = s0; j < s1; j++) {
+                                    float xq[D];
+                                    if (dist2_to<D>(x, Xs, n, j, xq) <= thr_half) half++;
+                                }
+                            }
+                        }
+                    } else if (included) {
+                        for (int j = s0; j < s1; j++) {
+                            float xq[D];
+                            if (dist2_to<D>(x, Xs, n, j, xq) <= thr) {
+#pragma unroll
+                                for (int l = 0; l < D; l++) sml[l] += sinf(xq[l] - x[l]);
+                                N++;
+                            }
+                        }
+                    }
+                }
+            }
+            odo.next(od, R);
+        }
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            float t = sum[l] + sml[l];
+            t = t / (float)N;
+            Xout[(long long)l * n + i] = x[l] + t;
+        }
+        perm_out[i] = perm_s[i];
+        const bool synced = !((!border && N != own) || (border && N != half));
+        if (!synced) unsynced = true;
+        if (dbg.N) dbg.N[i] = N;
+        if (dbg.half) dbg.half[i] = half;
+        if (dbg.border) dbg.border[i] = border ? 1 : 0;
+        if (dbg.synced) dbg.synced[i] = synced ? 1 : 0;
+    }
+    if (__any(unsynced) && __lane_id() == 0) atomicAnd(&st->flag, 0);
+}
+
+// MBR_in_eps (EGG_SynC.cu:1037-1159) for the pair (p, q): per dimension the
+// coordinate, among q and q's eps/2-neighbours, closest to p; true if that
+// point is within eps of p.
+template <int D>
+__device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__restrict__ Xs,
+                           const int *__restrict__ cell_start, const int *__restrict__ cell_g,
+                           const int *__restrict__ ob, const int *__restrict__ oe,
+                           const Geo &g) {
+    const int n = g.n, od = g.od, R = g.R_half;
+    float mbr[D];
+    int g0[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        mbr[l] = xq[l];
+        g0[l] = cell_coord(xq[l], g.cs, g.w);
+    }
+    const int g0od = select_dim<D>(g0, od);
+    const int *cg_od = cell_g + (long long)od * n;
+    Odo<D> odo;
+    odo.init(od, R);
+    while (!odo.finished) {
+        const int o = odo.outer(g0, g);
+        if (o >= 0) {
+            int cb = ob[o];
+            const int ce = oe[o];
+            if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+            for (int c = cb; c < ce; c++) {
+                int gc[D];
+#pragma unroll
+                for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
+                if (od < D && select_dim<D>(gc, od) > g0od + R + 1) break;
+                bool skip = false;
+#pragma unroll
+                for (int l = 0; l < D; l++)
+                    if (l > od && (gc[l] - g0[l] > R + 1 || g0[l] - gc[l] > R + 1)) skip = true;
+                if (skip) continue;
+                float sm, lg;
+                box_dist<D>(gc, xq, g.cs, sm, lg);
+                if (!(sm <= g.thr_half)) continue;
+                for (int j = cell_start[c]; j < cell_start[c + 1]; j++) {
+                    float x2[D];
+                    if (dist2_to<D>(xq, Xs, n, j, x2) <= g.thr_half) {
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            float od_ = mbr[l] - xp[l];
+                            od_ *= od_;
+                            float nd = x2[l] - xp[l];
+                            nd *= nd;
+                            if (nd < od_) mbr[l] = x2[l];
+                        }
+                    }
+                }
+            }
+        }
+        odo.next(od, R);
+    }
+    float dd = 0.f;
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        float t = mbr[l] - xp[l];
+        t *= t;
+        dd += t;
+    }
+    return dd <= g.thr_eps;
+}
+
+// kernel_float_grid_extra_check_part1 + part2 (EGG_SynC.cu:767-1034,1162-1203),
+// fused: each point looks for partners q with eps < |p-q| <= eps_extra in
+// cells that are not fully within eps, and tests the MBR of (p, q) at once.
+// Runs only when every point is locally synced; any hit clears the flag.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_extra_check(
+    const float *__restrict__ Xs, const int *__restrict__ cell_start,
+    const int *__restrict__ cell_g, const int *__restrict__ ob, const int *__restrict__ oe,
+    Geo g, DevState *st) {
+    if (st->done || st->flag == 0) return;
+    const int n = g.n, od = g.od, R = g.R_extra;
+    const int *cg_od = cell_g + (long long)od * n;
+    bool hit = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n && !hit;
+         i += gridDim.x * blockDim.x) {
+        if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+        float xp[D];
+        int g0[D];
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            xp[l] = Xs[(long long)l * n + i];
+            g0[l] = cell_coord(xp[l], g.cs, g.w);
+        }
+        const int g0od = select_dim<D>(g0, od);
+        Odo<D> odo;
+        odo.init(od, R);
+        while (!odo.finished && !hit) {
+            const int o = odo.outer(g0, g);
+            if (o >= 0) {
+                int cb = ob[o];
+                const int ce = oe[o];
+                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+                for (int c = cb; c < ce && !hit; c++) {
+                    int gc[D];
+#pragma unroll
+                    for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
+                    if (od < D && select_dim<D>(gc, od) > g0od + R + 1) break;
+                    bool skip = false;
+#pragma unroll
+                    for (int l = 0; l < D; l++)
+                        if (l > od && (gc[l] - g0[l] > R + 1 || g0[l] - gc[l] > R + 1)) skip = true;
+                    if (skip) continue;
+                    float sm, lg;
+                    box_dist<D>(gc, xp, g.cs, sm, lg);
+                    const bool included = sm <= g.thr_extra;
+                    const bool full = lg <= g.thr_eps;
+                    if (full || !included) continue;
+                    for (int j = cell_start[c]; j < cell_start[c + 1]; j++) {
+                        float xq[D];
+                        const float s = dist2_to<D>(xp, Xs, n, j, xq);
+                        if (!(s <= g.thr_eps) && s <= g.thr_extra) {
+                            if (mbr_in_eps<D>(xp, xq, Xs, cell_start, cell_g, ob, oe, g)) {
+                                hit = true;
+                                break;
+                            }
+                        }
+                    }
+                }
+            }
+            odo.next(od, R);
+        }
+    }
+    if (hit) atomicAnd(&st->flag, 0);
+}
+
+// SynC pass over the grid (SynC.cpp:16-56): every point within FP32 distance
+// eps (candidates: Chebyshev radius R+1 in all dims, which contains every such
+// point) contributes sinf(x_j - x) to the update and expf(-|x_j - x|) to r_p.
+// r_p and the block partial sums of r are FP64; the partials are combined in
+// a fixed order by k_rc_finish, so r_c is deterministic.
+constexpr int kRcBlocks = 1024;
+
+template <int D, bool WRITE>
+__global__ __launch_bounds__(kBlock) void k_update_sync(
+    const float *__restrict__ Xs, float *__restrict__ Xout, const int *__restrict__ perm_s,
+    int *__restrict__ perm_out, const int *__restrict__ point_cell,
+    const int *__restrict__ cell_start, const int *__restrict__ cell_g,
+    const int *__restrict__ ob, const int *__restrict__ oe, Geo g, DevState *st,
+    double *__restrict__ rc_part) {
+    if (WRITE && st->done) return;
+    const int n = g.n, od = g.od, R = g.R + 1;
+    const int *cg_od = cell_g + (long long)od * n;
+    double racc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float x[D], sum[D];
+        int g0[D];
+        const int c0 = point_cell[i];
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            x[l] = Xs[(long long)l * n + i];
+            g0[l] = cell_g[(long long)l * n + c0];
+            sum[l] = 0.f;
+        }
+        const int g0od = select_dim<D>(g0, od);
+        int N = 0;
+        double r = 0.0;
+        Odo<D> odo;
+        odo.init(od, R);
+        while (!odo.finished) {
+            const int o = odo.outer(g0, g);
+            if (o >= 0) {
+                int cb = ob[o];
+                const int ce = oe[o];
+                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R);
+                for (int c = cb; c < ce; c++) {
+                    int gc[D];
+#pragma unroll
+                    for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
+                    if (od < D && select_dim<D>(gc, od) > g0od + R) break;
+                    bool skip = false;
+#pragma unroll
+                    for (int l = 0; l < D; l++)
+                        if (l > od && (gc[l] - g0[l] > R || g0[l] - gc[l] > R)) skip = true;
+                    if (skip) continue;
+                    for (int j = cell_start[c]; j < cell_start[c + 1]; j++) {
+                        float xq[D];
+                        const float s = dist2_to<D>(x, Xs, n, j, xq);
+                        if (s <= g.thr_eps) {
+                            if (WRITE) {
+#pragma unroll
+                                for (int l = 0; l < D; l++) sum[l] += sinf(xq[l] - x[l]);
+                            }
+                            N++;
+                            r += (double)expf(-fabsf(sqrtf(s)));
+                        }
+                    }
+                }
+            }
+            odo.next(od, R);
+        }
+        if (WRITE) {
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                const float t = sum[l] / (float)N;
+                Xout[(long long)l * n + i] = x[l] + t;
+            }
+            perm_out[i] = perm_s[i];
+        }
+        racc += N > 0 ? r / (double)N : 1.0;
+    }
+    // deterministic block reduction
+    __shared__ double s_r[kBlock / kWave];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) racc += __shfl_xor(racc, o, kWave);
+    if (__lane_id() == 0) s_r[threadIdx.x / kWave] = racc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; w++) t += s_r[w];
+        rc_part[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rc_finish(const double *__restrict__ rc_part,
+                                                      int nparts, int n, DevState *st,
+                                                      bool guard) {
+    if (guard && st->done) return;
+    __shared__ double s_r[kBlock];
+    double t = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += kBlock) t += rc_part[k];
+    s_r[threadIdx.x] = t;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) s_r[threadIdx.x] += s_r[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) st->rc = s_r[0] / (double)n;
+}
+
+__global__ void k_set_flag(DevState *st) {
+    if (st->done) return;
+    st->flag = 1;
+}
+
+// End of an iteration (EGG_SynC.cu:1322,1425 / SynC.cpp:277).
+__global__ void k_finish(DevState *st) {
+    if (st->done) return;
+    st->iter += 1;
+    if (st->stop_rule == 0) {
+        if (st->flag) st->done = 1;
+    } else {
+        if (st->rc >= (double)st->lam) st->done = 1;
+    }
+    if (!st->done && st->max_iter > 0 && st->iter >= st->max_iter) {
+        st->done = 1;
+        st->capped = 1;
+    }
+}
+
+static inline int grid_for(long long n) {
+    long long b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > 65536) b = 65536;
+    return (int)b;
+}
+
+void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *dbg,
+                       hipStream_t s) {
+    DebugOut d = dbg ? *dbg : DebugOut{nullptr, nullptr, nullptr, nullptr};
+    const int gb = grid_for(g.n);
+#define UPD_CASE(DD)                                                                          \
+    case DD:                                                                                  \
+        k_update_egg<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.X, b.perm_s, b.perm, b.point_cell,     \
+                                               b.cell_start, b.cell_g, b.Ts, b.Tc, b.ob,      \
+                                               b.oe, g, st, d);                               \
+        break;
+    switch (g.d) {
+        UPD_CASE(1) UPD_CASE(2) UPD_CASE(3) UPD_CASE(4)
+        UPD_CASE(5) UPD_CASE(6) UPD_CASE(7) UPD_CASE(8)
+    }
+#undef UPD_CASE
+}
+
+void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s) {
+    const int gb = grid_for(g.n);
+#define EX_CASE(DD)                                                                       \
+    case DD:                                                                              \
+        k_extra_check<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.ob, b.oe, \
+                                                g, st);                                   \
+        break;
+    switch (g.d) {
+        EX_CASE(1) EX_CASE(2) EX_CASE(3) EX_CASE(4)
+        EX_CASE(5) EX_CASE(6) EX_CASE(7) EX_CASE(8)
+    }
+#undef EX_CASE
+}
+
+int rc_partials(int n) { return kRcBlocks; }
+
+void launch_update_sync(const Geo &g, GridBufs &b, DevState *st, double *rc_part, bool write_pos,
+                        hipStream_t s) {
+#define SY_CASE(DD)                                                                          \
+    case DD:                                                                                 \
+        if (write_pos)                                                                       \
+            k_update_sync<DD, true><<<kRcBlocks, kBlock, 0, s>>>(                            \
+                b.Xs, b.X, b.perm_s, b.perm, b.point_cell, b.cell_start, b.cell_g, b.ob,     \
+                b.oe, g, st, rc_part);                                                       \
+        else                                                                                 \
+            k_update_sync<DD, false><<<kRcBlocks, kBlock, 0, s>>>(                           \
+                b.Xs, nullptr, b.perm_s, nullptr, b.point_cell, b.cell_start, b.cell_g,      \
+                b.ob, b.oe, g, st, rc_part);                                                 \
+        break;
+    switch (g.d) {
+        SY_CASE(1) SY_CASE(2) SY_CASE(3) SY_CASE(4)
+        SY_CASE(5) SY_CASE(6) SY_CASE(7) SY_CASE(8)
+    }
+#undef SY_CASE
+    k_rc_finish<<<1, kBlock, 0, s>>>(rc_part, kRcBlocks, g.n, st, write_pos);
+}
+
+void launch_set_flag(DevState *st, hipStream_t s) { k_set_flag<<<1, 1, 0, s>>>(st); }
+
+void launch_finish(DevState *st, hipStream_t s) { k_finish<<<1, 1, 0, s>>>(st); }
+
+}  // namespace egg

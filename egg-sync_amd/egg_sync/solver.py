@@ -1,0 +1,108 @@
+"""Thin Python wrappers of the C ABI for device-resident data (torch tensors on the GPU).
+
+These call libeggsync.so through ctypes; torch only provides device memory and
+the current stream.  Used by the tests, the benchmark and callers that want
+statistics or intermediate results of the solve.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+
+def _dev_f32(X, device="cuda"):
+    if isinstance(X, torch.Tensor):
+        return X.to(device=device, dtype=torch.float32).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).to(device)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-3, max_iter=0,
+          compute_rc=False, timing=False, iters_per_sync=4, return_positions=True,
+          itr_cap=16384):
+    """Solve on the GPU.  Returns dict(labels int32 numpy, X final numpy or None, stats)."""
+    D = _dev_f32(X)
+    n, d = D.shape
+    if label_rule is None:
+        label_rule = nat.LABELS_SYNC if stop_rule == nat.STOP_RC else nat.LABELS_CELLS
+    opt = nat.default_options(stop_rule=stop_rule, label_rule=label_rule, lam=lam,
+                              max_iter=max_iter, compute_rc=int(compute_rc), timing=int(timing),
+                              iters_per_sync=iters_per_sync)
+    labels = torch.empty(n, dtype=torch.int32, device=D.device)
+    final = torch.empty_like(D) if return_positions else None
+    st = nat.Stats()
+    itr = np.zeros(itr_cap, np.float32)
+    st.itr_ms = itr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    st.itr_ms_cap = itr_cap
+    rc = nat.lib().egg_sync_solve_device(nat.ptr(D), n, d, float(eps), ctypes.byref(opt),
+                                         nat.ptr(labels), nat.ptr(final), ctypes.byref(st),
+                                         _stream())
+    nat.check(rc, "egg_sync_solve_device")
+    stats = {k: getattr(st, k) for k in ("iterations", "status", "n_clusters", "n_cells", "rc",
+                                         "total_ms", "update_kernel_ms", "update_launches",
+                                         "workspace_bytes")}
+    stats["stage_ms"] = list(st.stage_ms)
+    stats["itr_ms"] = itr[: st.iterations].copy() if timing else None
+    return dict(labels=labels.cpu().numpy(), X=None if final is None else final.cpu().numpy(),
+                stats=stats, code=rc)
+
+
+def build_grid(X, eps: float, order_in=None):
+    """Grid build on the GPU; same outputs as the CPU oracle's build_grid."""
+    D = _dev_f32(X)
+    n, d = D.shape
+    g = nat.geometry(n, d, eps)
+    dev = D.device
+    oi = None if order_in is None else torch.as_tensor(np.asarray(order_in, np.int32)).to(dev)
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    cell_of = torch.empty(n, dtype=torch.int32, device=dev)
+    cell_start = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    cell_g = torch.empty(n * d, dtype=torch.int32, device=dev)
+    ob = torch.empty(g["O"], dtype=torch.int32, device=dev)
+    oe = torch.empty(g["O"], dtype=torch.int32, device=dev)
+    C = ctypes.c_int32()
+    rc = nat.lib().egg_sync_build_grid_device(nat.ptr(D), n, d, float(eps), nat.ptr(oi),
+                                              nat.ptr(order), nat.ptr(cell_of),
+                                              nat.ptr(cell_start), nat.ptr(cell_g), nat.ptr(ob),
+                                              nat.ptr(oe), ctypes.byref(C), _stream())
+    nat.check(rc, "egg_sync_build_grid_device")
+    C = C.value
+    return dict(order=order.cpu().numpy(), cell_of=cell_of.cpu().numpy(),
+                cell_start=cell_start[: C + 1].cpu().numpy(),
+                cell_g=cell_g[: C * d].cpu().numpy().reshape(C, d), ob=ob.cpu().numpy(),
+                oe=oe.cpu().numpy(), C=C)
+
+
+def iteration(X, eps: float, order_in=None):
+    """One EGG iteration on the GPU; same outputs as the CPU oracle's egg_iteration."""
+    D = _dev_f32(X)
+    n, d = D.shape
+    dev = D.device
+    oi = None if order_in is None else torch.as_tensor(np.asarray(order_in, np.int32)).to(dev)
+    Xn = torch.empty_like(D)
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    N, half, border, synced = (torch.empty(n, dtype=torch.int32, device=dev) for _ in range(4))
+    fl, ff = ctypes.c_int32(), ctypes.c_int32()
+    rc = nat.lib().egg_sync_iteration_device(nat.ptr(D), n, d, float(eps), nat.ptr(oi),
+                                             nat.ptr(Xn), nat.ptr(order), nat.ptr(N),
+                                             nat.ptr(half), nat.ptr(border), nat.ptr(synced),
+                                             ctypes.byref(fl), ctypes.byref(ff), _stream())
+    nat.check(rc, "egg_sync_iteration_device")
+    return dict(X_next=Xn.cpu().numpy(), order=order.cpu().numpy(), N=N.cpu().numpy(),
+                half=half.cpu().numpy(), border=border.cpu().numpy(),
+                synced=synced.cpu().numpy(), flag_local=fl.value, flag_final=ff.value)
+
+
+def rc(X, eps: float) -> float:
+    D = _dev_f32(X)
+    out = ctypes.c_double()
+    nat.check(nat.lib().egg_sync_rc_device(nat.ptr(D), D.shape[0], D.shape[1], float(eps),
+                                           ctypes.byref(out), _stream()), "egg_sync_rc_device")
+    return out.value

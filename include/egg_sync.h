@@ -1,0 +1,138 @@
+/*
+ * egg_sync.h -- C ABI of the MI355X (gfx950) EGG-SynC solver.
+ *
+ * Plain C types only (no HIP or torch types): a HIP stream is passed as
+ * `void *` (hipStream_t), device buffers as plain pointers.  Every entry point
+ * returns 0 on success, a positive status for a completed solve that hit the
+ * iteration cap, or a negative EGG_SYNC_E* error code; nothing calls exit().
+ *
+ * Reference interfaces replaced (jakobrj/EGG-SynC):
+ *   egg_sync_solve           std::vector<at::Tensor> EGG_SynC(float *h_D, int n, int d, float eps)
+ *                            (src/algorithms/EGG_SynC.cuh:11, src/algorithms/EGG_SynC.cu:1229),
+ *                            reached from Python through EGG_SynC_wrapper (src/maps/map.cpp:31-33)
+ *                            and python/algorithms.py:110-111.  With stop_rule
+ *                            EGG_SYNC_STOP_RC it also replaces DynamicalClustering
+ *                            (src/algorithms/SynC.h:38, src/maps/map.cpp:16-18) on the GPU.
+ *   egg_sync_solve_device    the same solve on device-resident data (no reference counterpart:
+ *                            the reference always copies a host array in, EGG_SynC.cu:1245).
+ *   egg_sync_build_grid_device   build_the_grid (EGG_SynC.cu:248-341)
+ *   egg_sync_iteration_device    one pass of the loop body (EGG_SynC.cu:1322-1491)
+ *   egg_sync_rc_device           order parameter r_c (SynC.cpp:42-56, FP64 sums)
+ * Threading: one solve at a time per process (the library caches one
+ * workspace per device, like the reference's global allocation map,
+ * GPU_utils.cu:176-178); calls are synchronous with respect to the host.
+ */
+#ifndef EGG_SYNC_H
+#define EGG_SYNC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EGG_SYNC_OK 0
+#define EGG_SYNC_CAPPED 1           /* solved, but max_iter stopped the loop   */
+#define EGG_SYNC_E_ARG (-1)         /* bad n / d / eps / pointer               */
+#define EGG_SYNC_E_RANGE (-2)       /* input outside [0,1] or not finite       */
+#define EGG_SYNC_E_DIM (-3)         /* d above the supported maximum           */
+#define EGG_SYNC_E_KEY (-4)         /* cell keys would need more than 128 bits */
+#define EGG_SYNC_E_ALLOC (-5)       /* device allocation failed                */
+#define EGG_SYNC_E_HIP (-6)         /* HIP runtime error                       */
+#define EGG_SYNC_E_NODEV (-7)       /* no GPU visible                          */
+
+#define EGG_SYNC_STOP_EGG 0         /* local sync + MBR extra check (EGG_SynC.cu:706-709,1425-1489) */
+#define EGG_SYNC_STOP_RC 1          /* r_c >= lam (SynC.cpp:277)                                    */
+#define EGG_SYNC_LABELS_CELLS 0     /* EGG_SynC.cu:1205-1227 */
+#define EGG_SYNC_LABELS_SYNC 1      /* synCluster_v2, SynC.cpp:88-129 */
+
+#define EGG_SYNC_MAX_D 8
+
+typedef struct egg_sync_options {
+    int32_t stop_rule;       /* EGG_SYNC_STOP_*                                   */
+    int32_t label_rule;      /* EGG_SYNC_LABELS_*                                 */
+    float lam;               /* threshold of EGG_SYNC_STOP_RC (reference 1-1e-3)  */
+    int32_t max_iter;        /* 0 = no cap (the reference has none)               */
+    int32_t compute_rc;      /* 1: report r_c (FP64) of the final positions       */
+    int32_t timing;          /* 1: per-stage / per-iteration GPU event timings    */
+    int32_t iters_per_sync;  /* iterations enqueued between host checks (>= 1)    */
+    int32_t validate;        /* 1: check the input lies in [0,1] (default 1)      */
+} egg_sync_options;
+
+typedef struct egg_sync_stats {
+    int32_t iterations;      /* passes of the update loop                          */
+    int32_t status;          /* 0 converged, 1 capped                              */
+    int32_t n_clusters;
+    int32_t n_cells;         /* non-empty cells of the final grid                  */
+    double rc;               /* FP64 order parameter (NaN unless computed)         */
+    double stage_ms[6];      /* Allocating, Build structure, Update, Extra check,
+                                Clustering, Free Memory (run_experiment.py:20)     */
+    double total_ms;         /* wall time of the call                              */
+    double update_kernel_ms; /* summed duration of the update kernel (timing=1)    */
+    int32_t update_launches;
+    int32_t pad;
+    int64_t workspace_bytes; /* device bytes held for the solve ("space")          */
+    float *itr_ms;           /* optional caller array, per-iteration GPU time      */
+    int32_t itr_ms_cap;      /* capacity of itr_ms                                 */
+    int32_t pad2;
+} egg_sync_stats;
+
+/* Fills *opt with the defaults of EGG_SynC (stop_rule EGG, labels cells, lam 0.999). */
+int egg_sync_default_options(egg_sync_options *opt);
+
+/* Host arrays in and out: h_D is n x d row-major float32 in [0,1];
+ * labels_out[n] int32 (-1 = noise).  stats_out may be NULL. */
+int egg_sync_solve(const float *h_D, int64_t n, int32_t d, float eps, int32_t stop_rule,
+                   int32_t *labels_out, egg_sync_stats *stats_out);
+
+/* As egg_sync_solve with options and optional final positions (n x d, host). */
+int egg_sync_solve_ex(const float *h_D, int64_t n, int32_t d, float eps,
+                      const egg_sync_options *opt, int32_t *labels_out, float *final_out,
+                      egg_sync_stats *stats_out);
+
+/* Device arrays: d_D (n x d row-major), d_labels[n], d_final (n x d, may be
+ * NULL), all on the current device; work is ordered on `stream`. */
+int egg_sync_solve_device(const float *d_D, int64_t n, int32_t d, float eps,
+                          const egg_sync_options *opt, int32_t *d_labels, float *d_final,
+                          egg_sync_stats *stats_out, void *stream);
+
+/* Grid build on device positions d_D (n x d) visited in order d_order_in
+ * (point ids; NULL = identity).  Outputs (device, may be NULL):
+ * d_order_out[n] point id per sorted position, d_cell_of[n],
+ * d_cell_start[n+1], d_cell_g[n*d] (cell-major, C x d), d_ob[O], d_oe[O];
+ * *C_out (host) = number of non-empty cells. */
+int egg_sync_build_grid_device(const float *d_D, int64_t n, int32_t d, float eps,
+                               const int32_t *d_order_in, int32_t *d_order_out,
+                               int32_t *d_cell_of, int32_t *d_cell_start, int32_t *d_cell_g,
+                               int32_t *d_ob, int32_t *d_oe, int32_t *C_out, void *stream);
+
+/* One iteration of the EGG loop on device positions d_D (n x d, point-id
+ * order) visited in order d_order_in (NULL = identity): writes d_next (n x d,
+ * point-id order), d_order_out[n] and, per sorted position, d_N, d_half,
+ * d_border, d_synced (each may be NULL); *flag_local / *flag_final (host) as
+ * the reference's r_local before / after the extra check. */
+int egg_sync_iteration_device(const float *d_D, int64_t n, int32_t d, float eps,
+                              const int32_t *d_order_in, float *d_next, int32_t *d_order_out,
+                              int32_t *d_N, int32_t *d_half, int32_t *d_border,
+                              int32_t *d_synced, int32_t *flag_local, int32_t *flag_final,
+                              void *stream);
+
+/* r_c of device positions d_D (n x d): FP32 eps-membership and expf terms,
+ * FP64 sums (written to *rc_out on the host). */
+int egg_sync_rc_device(const float *d_D, int64_t n, int32_t d, float eps, double *rc_out,
+                       void *stream);
+
+/* Grid geometry the solver derives from (n, d, eps) (EGG_SynC.cu:1249-1260):
+ * out[0]=w, out[1]=od, out[2]=O, out[3]=R, out[4]=key bits; *cs_out cell side. */
+int egg_sync_geometry(int64_t n, int32_t d, float eps, int64_t *out5, float *cs_out);
+
+/* Frees the cached device workspace of the current device. */
+int egg_sync_release(void);
+
+/* Static description of an error code. */
+const char *egg_sync_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EGG_SYNC_H */

@@ -1,0 +1,240 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: integer results (sorted order, cells, ranges, neighbour counts, flags,
+iteration counts) bit-exact; labels identical partitions; FP32 positions
+within 1e-6 after one pass and 1e-5 after a full solve.  Full-size runs are
+checked through size-independent properties (determinism, bit-exact pass on
+the converged state, labels equal to the cell rule on the final positions).
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+f32 = np.float32
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def S():
+    assert torch.cuda.is_available()
+    from egg_sync import solver
+    return solver
+
+
+def _cases_grid():
+    from egg_sync import data
+    rng = np.random.default_rng(7)
+    cases = [("c0", *data.config0()),
+             ("c1_20k", data.config1(n=20000)[0], 0.02),
+             ("c2_5k", data.config2(n=5000)[0], 0.1)]
+    for d, eps in [(1, 0.01), (4, 0.1), (5, 0.15), (6, 0.2), (7, 0.3), (8, 0.01)]:
+        X = rng.random((3000, d)).astype(f32)
+        X[:3] = 1.0
+        X[3:6] = 0.0
+        cases.append((f"rand_d{d}", X, eps))
+    return cases
+
+
+@pytest.mark.parametrize("case", _cases_grid(), ids=lambda c: c[0])
+def test_grid_bit_exact(S, O, case):
+    name, X, eps = case
+    _say("grid", name, X.shape, eps)
+    rng = np.random.default_rng(3)
+    for order_in in (None, rng.permutation(len(X)).astype(np.int32)):
+        a = S.build_grid(X, eps, order_in)
+        b = O.build_grid(X, eps, order_in)
+        assert a["C"] == b["C"]
+        for k in ("order", "cell_of", "cell_start", "cell_g", "ob", "oe"):
+            assert np.array_equal(a[k], b[k]), k
+
+
+def _cases_iter():
+    from egg_sync import data
+    cases = [(nm, golden(nm)["X"], float(golden(nm)["eps"]))
+             for nm in ("step_c0", "step_c0_mid", "step_d3", "step_d5", "step_d8")]
+    cases.append(("c1_30k", data.config1(n=30000)[0], 0.02))
+    cases.append(("c2_4k", data.config2(n=4000)[0], 0.1))
+    X0, eps0 = data.config0()
+    cases.append(("c0_tight", X0, 0.01))
+    return cases
+
+
+@pytest.mark.parametrize("case", _cases_iter(), ids=lambda c: c[0])
+def test_iteration_bit_exact(S, O, case):
+    name, X, eps = case
+    _say("iteration", name, X.shape, eps)
+    a = S.iteration(X, eps)
+    b = O.egg_iteration(X, eps)
+    for k in ("order", "N", "half", "border", "synced"):
+        assert np.array_equal(a[k], b[k]), k
+    assert (a["flag_local"], a["flag_final"]) == (b["flag_local"], b["flag_final"])
+    err = float(np.abs(a["X_next"] - b["X_next"]).max())
+    assert err <= 1e-6, err
+
+
+def _advance(O, X, eps, k):
+    for _ in range(k):
+        X = O.egg_iteration(X, eps)["X_next"]
+    return X
+
+
+def test_iteration_bit_exact_late_states(S, O):
+    """States after several passes: collapsing clusters, border points, extra check."""
+    from egg_sync import data
+    X, eps = data.config0()
+    for k in (2, 5, 8):
+        Xk = _advance(O, X, eps, k)
+        _say("late state", k)
+        a = S.iteration(Xk, eps)
+        b = O.egg_iteration(Xk, eps)
+        for key in ("order", "N", "half", "border", "synced"):
+            assert np.array_equal(a[key], b[key]), (k, key)
+        assert (a["flag_local"], a["flag_final"]) == (b["flag_local"], b["flag_final"])
+        assert np.abs(a["X_next"] - b["X_next"]).max() <= 1e-6
+
+
+def _cases_solve():
+    from egg_sync import data
+    return [("c0", *data.config0()),
+            ("c0_eps02", data.config0()[0], 0.02),
+            ("c1_20k", data.config1(n=20000)[0], 0.02),
+            ("c2_5k", data.config2(n=5000)[0], 0.1),
+            ("d1", data.gaussian_blobs(2000, 1, 5, seed=9, center_lo=0.1, center_hi=0.9,
+                                       sigma=0.02, noise=0.05), 0.02),
+            ("d5", data.gaussian_blobs(3000, 5, 6, seed=11, center_lo=0.2, center_hi=0.8,
+                                       sigma=0.05, noise=0.05), 0.15)]
+
+
+@pytest.mark.parametrize("case", _cases_solve(), ids=lambda c: c[0])
+def test_solve_matches_oracle(S, O, case):
+    name, X, eps = case
+    _say("solve", name, X.shape, eps)
+    a = S.solve(X, eps)
+    b = O.egg_solve(X, eps)
+    assert a["stats"]["iterations"] == b["iterations"]
+    assert a["stats"]["status"] == 0
+    assert O.same_partition(a["labels"], b["labels"])
+    assert np.abs(a["X"] - b["X"]).max() <= 1e-5
+    assert a["stats"]["n_clusters"] == len(set(b["labels"][b["labels"] >= 0]))
+
+
+def test_sync_mode_matches_reference_golden(S, O):
+    g = golden("c0_sync")
+    _say("sync mode c0")
+    from egg_sync import _native as nat
+    a = S.solve(g["X"], float(g["eps"]), stop_rule=nat.STOP_RC, lam=float(g["lam"]))
+    assert a["stats"]["iterations"] == int(g["iterations"])
+    assert O.same_partition(a["labels"], g["labels"])
+    assert np.array_equal(a["labels"], g["labels"])  # same greedy numbering too
+    assert np.abs(a["X"] - g["X_final"]).max() <= 1e-6
+    assert abs(a["stats"]["rc"] - float(g["r_trace"][-1])) <= 1e-6
+
+
+def test_rc_matches_oracle(S, O):
+    from egg_sync import data
+    X, eps = data.config0()
+    Xk = _advance(O, X, eps, 3)
+    for Y in (X, Xk):
+        assert abs(S.rc(Y, eps) - O.rc(Y, eps)) <= 1e-7
+
+
+def test_edge_cases(S, O):
+    from egg_sync import _native as nat
+    _say("edge cases")
+    r = S.solve(np.array([[0.25, 0.75]], f32), 0.05)
+    assert list(r["labels"]) == [-1] and r["stats"]["iterations"] == 1
+    X = np.tile(np.array([[0.4, 0.6, 0.1]], f32), (9, 1))
+    r = S.solve(X, 0.05)
+    assert len(set(r["labels"])) == 1 and r["labels"][0] >= 0
+    X = np.array([[0.0, 0.0], [1.0, 1.0], [0.0, 1.0], [1.0, 0.0], [1.0, 1.0]], f32)
+    a, b = S.solve(X, 0.05), O.egg_solve(X, 0.05)
+    assert O.same_partition(a["labels"], b["labels"])
+    # eps so large that the grid is a single cell (w clamps to 1)
+    rng = np.random.default_rng(1)
+    X = rng.random((200, 2)).astype(f32)
+    a, b = S.solve(X, 3.0), O.egg_solve(X, 3.0)
+    assert a["stats"]["iterations"] == b["iterations"]
+    assert O.same_partition(a["labels"], b["labels"])
+    # iteration cap comes back as a status, not an exit
+    from egg_sync import data
+    Xc, eps = data.config0()
+    r = S.solve(Xc, eps, max_iter=1)
+    assert r["code"] == nat.CAPPED and r["stats"]["status"] == 1 and r["stats"]["iterations"] == 1
+    # invalid input
+    for bad in (np.array([[0.5, 1.5]], f32), np.array([[np.nan, 0.1]], f32),
+                np.array([[-0.1, 0.1]], f32)):
+        with pytest.raises(nat.EggSyncError) as e:
+            S.solve(bad, 0.05)
+        assert e.value.code == nat.E_RANGE
+    with pytest.raises(nat.EggSyncError):
+        S.solve(np.zeros((4, 9), f32), 0.05)
+
+
+def test_deterministic(S):
+    from egg_sync import data
+    X, eps = data.config1(n=50000)
+    a, b = S.solve(X, eps), S.solve(X, eps)
+    assert np.array_equal(a["X"], b["X"]) and np.array_equal(a["labels"], b["labels"])
+    assert a["stats"]["iterations"] == b["stats"]["iterations"]
+
+
+def test_torch_extension_api(S, O):
+    from egg_sync import algorithms, data
+    X, eps = data.config0()
+    _say("torch extension")
+    for D in (torch.from_numpy(X), torch.from_numpy(X).cuda()):
+        labels, itr_times, stage_times, space = algorithms.EGG_SynC(D, eps)
+        assert labels.dtype == torch.float32 and labels.shape == (len(X),)
+        assert stage_times.shape == (6,) and space.numel() == 1 and int(space[0]) > 0
+        ref = O.egg_solve(X, eps)
+        assert itr_times.shape == (ref["iterations"],)
+        assert O.same_partition(labels.cpu().numpy().astype(np.int64), ref["labels"])
+    g = golden("c0_sync")
+    labels, itr_times, _, _ = algorithms.SynC(torch.from_numpy(g["X"]), float(g["eps"]))
+    assert labels.dtype == torch.int32
+    assert np.array_equal(labels.numpy(), g["labels"])
+    assert itr_times.shape == (int(g["iterations"]),)
+
+
+def test_full_size_config1_properties(S, O):
+    """n = 1,000,000, d = 3, eps = 0.02: convergence, determinism, and a
+    bit-exact pass of the converged state against the oracle at full size."""
+    from egg_sync import data
+    X, eps = data.config1()
+    _say("config1 full size solve")
+    a = S.solve(X, eps)
+    assert a["stats"]["status"] == 0
+    _say("config1 iterations", a["stats"]["iterations"], "clusters", a["stats"]["n_clusters"])
+    b = S.solve(X, eps)
+    assert np.array_equal(a["X"], b["X"]) and np.array_equal(a["labels"], b["labels"])
+    # labels are the cell rule applied to the final positions
+    assert O.same_partition(a["labels"], O.egg_labels(a["X"], eps))
+    # one more pass from the final state: GPU and oracle agree bit for bit
+    _say("config1 pass on the final state")
+    ga = S.iteration(a["X"], eps)
+    gb = O.egg_iteration(a["X"], eps)
+    for k in ("order", "N", "half", "border", "synced"):
+        assert np.array_equal(ga[k], gb[k]), k
+    assert (ga["flag_local"], ga["flag_final"]) == (gb["flag_local"], gb["flag_final"])
+    assert np.abs(ga["X_next"] - gb["X_next"]).max() <= 1e-6
+
+
+def test_config1_sample_solve_matches_oracle(S, O):
+    from egg_sync import data
+    X, eps = data.config1(n=100_000)
+    _say("config1 n=100k solve")
+    a = S.solve(X, eps)
+    b = O.egg_solve(X, eps)
+    assert a["stats"]["iterations"] == b["iterations"]
+    assert O.same_partition(a["labels"], b["labels"])
+    assert np.abs(a["X"] - b["X"]).max() <= 1e-5
