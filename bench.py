@@ -1,0 +1,162 @@
+"""Benchmark: plain EGG-SynC solve of n=1,000,000 d=3 points at eps=0.02 on one MI355X.
+
+A step is one complete solve (data tensor already in HBM -> labels in HBM).
+value = point-updates/s = n * iterations * steps / timed seconds; the
+time-to-clustering of one solve is ms_per_step.  The roofline object is for the
+neighbour-update kernel (the dominant kernel): algorithmic bytes per launch
+8*n*d + (12 + 8*d)*C (read x, write x', read each non-empty cell's start, end,
+count and d sin / d cos totals once), divided by its mean launch duration from
+HIP events recorded around every update launch on the solver's stream during
+the timed steps.  The CPU baseline is the reference's own CPU SynC
+(src/algorithms/SynC.cpp, compiled from source into oracle/_ref) running one
+OpenMP pass over a bounded sample of target points of the same data.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "egg-sync_amd"))
+
+METRIC = "time-to-clustering (s), point-updates/s; plain vs learned form, same labels"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(X, eps, target_seconds=12.0):
+    """One SynC pass (SynC.cpp:352-361) over a sample of targets of the full data set."""
+    import numpy as np
+    n, d = X.shape
+    threads = len(os.sched_getaffinity(0))
+    ref = os.path.join(ROOT, "oracle", "_ref", "sync_ref")
+    with tempfile.TemporaryDirectory() as td:
+        inp = os.path.join(td, "in.bin")
+        with open(inp, "wb") as f:
+            f.write(np.array([n, d], np.int32).tobytes())
+            f.write(np.array([eps, 1 - 1e-3], np.float32).tobytes())
+            f.write(np.ascontiguousarray(X, np.float32).tobytes())
+
+        def run_ref(t):
+            out = subprocess.run([ref, "time", inp, "0", str(t), str(threads)], check=True,
+                                 capture_output=True, text=True).stdout
+            kv = dict(p.split("=") for p in out.split())
+            return float(kv["seconds"])
+
+        kind = "reference"
+        try:
+            if not os.path.exists(ref):
+                raise FileNotFoundError(ref)
+            probe_t = 4 * threads
+            s = run_ref(probe_t)
+            targets = int(min(n, max(probe_t, probe_t * target_seconds / max(s, 1e-3))))
+            secs = run_ref(targets)
+        except Exception:  # compiled reference unavailable: the oracle's restatement
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            kind = "port"
+            probe_t = 4 * threads
+            t0 = time.perf_counter()
+            O.sync_iteration_range(X, eps, 0, probe_t, threads)
+            s = time.perf_counter() - t0
+            targets = int(min(n, max(probe_t, probe_t * target_seconds / max(s, 1e-3))))
+            t0 = time.perf_counter()
+            O.sync_iteration_range(X, eps, 0, targets, threads)
+            secs = time.perf_counter() - t0
+    return {"value": targets / secs, "unit": "point-updates/s", "cores": threads, "kind": kind,
+            "sample": f"one SynC pass (brute-force eps-neighbourhood, SynC.cpp:352-361) for the "
+                      f"first {targets} of {n} points against all {n} points of the same data, "
+                      f"{secs:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--iters-per-sync", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the solve is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from egg_sync import _native as nat
+    from egg_sync import data, solver
+
+    X, eps = data.config1(n=args.n)
+    n, d = X.shape
+    D = torch.from_numpy(X).cuda()
+    torch.cuda.synchronize()
+
+    def one(timing):
+        return solver.solve(D, eps, timing=timing, iters_per_sync=args.iters_per_sync,
+                            return_positions=False)
+
+    for _ in range(args.warmup):
+        one(True)
+    torch.cuda.synchronize()
+    results = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        results.append(one(True))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    iters = [r["stats"]["iterations"] for r in results]
+    labels0 = results[0]["labels"]
+    same = all(np.array_equal(r["labels"], labels0) for r in results)
+    updates = sum(n * it for it in iters)
+    value = updates / wall
+    # roofline of the update kernel over the timed launches
+    upd_ms = sum(r["stats"]["update_kernel_ms"] for r in results)
+    launches = sum(r["stats"]["update_launches"] for r in results)
+    cells = np.concatenate([r["stats"]["cells_trace"] for r in results]).astype(np.float64)
+    bytes_per_launch = float(np.mean(8.0 * n * d + (12.0 + 8.0 * d) * cells))
+    avg_launch_s = upd_ms / 1000.0 / max(launches, 1)
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    st = results[-1]["stats"]
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1000.0 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 20-blob Gaussian mixture (Zipf 1.1 sizes, per-axis sigma "
+                "U[0.005,0.02]) + 2% uniform noise, min-max normalised to [0,1]^3",
+        "config": {"workload": f"plain EGG-SynC solve, n={n}, d={d}, eps={eps}, inputs resident "
+                               "in HBM, labels left in HBM",
+                   "form": "plain", "n": n, "d": d, "eps": eps, "iterations": iters[0],
+                   "time_to_clustering_s": wall / args.steps,
+                   "clusters": st["n_clusters"], "labels_identical_across_steps": same,
+                   "stage_ms_last_step": [round(x, 3) for x in st["stage_ms"]],
+                   "iteration_ms_last_step":
+                       [round(float(x), 3) for x in (st["itr_ms"] if st["itr_ms"] is not None else [])],
+                   "cells_per_iteration_last_step": [int(c) for c in st["cells_trace"]]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_update_egg<3>", "launches": launches,
+                     "avg_launch_us": avg_launch_s * 1e6,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(X, eps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

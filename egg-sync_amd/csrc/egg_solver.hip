@@ -128,6 +128,8 @@ struct Layout {
     float *io = nullptr;      // [n*d] input / output staging
     double *rc_part = nullptr;
     double *rc_trace = nullptr;
+    int *cells_trace = nullptr;
+    int *it_before = nullptr;
     int *err = nullptr;
     size_t bytes = 0;
 };
@@ -166,6 +168,8 @@ Layout carve(void *base, const Geo &g) {
     L.io = c.take<float>(n * d);
     L.rc_part = c.take<double>((size_t)rc_partials(g.n));
     L.rc_trace = c.take<double>(kTraceCap);
+    L.cells_trace = c.take<int>(kTraceCap);
+    L.it_before = c.take<int>(1);
     L.err = c.take<int>(1);
     L.bytes = c.off;
     return L;
@@ -231,10 +235,19 @@ __global__ void k_snapshot_flag(DevState *st) {
     st->flag_local = st->flag;
 }
 
-__global__ void k_trace_rc(DevState *st, double *trace, int cap) {
-    // called after k_finish: iteration `iter` (1-based) just completed
+__global__ void k_trace(const DevState *st, const int *it_before, double *rc_trace,
+                        int *cells_trace, int cap) {
+    // after k_finish: record r_c and C of the iteration that just ran
     const int it = st->iter;
-    if (it >= 1 && it <= cap) trace[it - 1] = st->rc;
+    if (it == *it_before) return;  // the loop had already stopped
+    if (it >= 1 && it <= cap) {
+        rc_trace[it - 1] = st->rc;
+        cells_trace[it - 1] = st->C;
+    }
+}
+
+__global__ void k_iter_snapshot(const DevState *st, int *it_before) {
+    *it_before = st->iter;
 }
 
 int grid_blocks(long long n) {
@@ -292,6 +305,7 @@ double ms_between(hipEvent_t a, hipEvent_t b) {
 void enqueue_iteration(const Geo &g, Layout &L, const egg_sync_options &opt, hipStream_t s,
                        hipEvent_t *ev) {
     if (ev) (void)hipEventRecord(ev[0], s);
+    k_iter_snapshot<<<1, 1, 0, s>>>(L.st, L.it_before);
     launch_set_flag(L.st, s);
     build_grid(g, L.b, L.st, L.st, s);
     if (opt.stop_rule == EGG_SYNC_STOP_EGG) cell_summaries(g, L.b, L.st, L.st, s);
@@ -306,7 +320,7 @@ void enqueue_iteration(const Geo &g, Layout &L, const egg_sync_options &opt, hip
         if (ev) (void)hipEventRecord(ev[2], s);
     }
     launch_finish(L.st, s);
-    k_trace_rc<<<1, 1, 0, s>>>(L.st, L.rc_trace, kTraceCap);
+    k_trace<<<1, 1, 0, s>>>(L.st, L.it_before, L.rc_trace, L.cells_trace, kTraceCap);
     if (ev) (void)hipEventRecord(ev[3], s);
 }
 
@@ -419,6 +433,14 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
                                hipMemcpyDeviceToHost, s));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ws->h_state, L.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+    const int ntr = so && so->trace_cap > 0 ? std::min(std::min(so->trace_cap, kTraceCap),
+                                                        loop_state.iter) : 0;
+    if (ntr > 0 && so->rc_trace)
+        HIP_TRY(hipMemcpyAsync(so->rc_trace, L.rc_trace, sizeof(double) * ntr,
+                               hipMemcpyDeviceToHost, s));
+    if (ntr > 0 && so->cells_trace)
+        HIP_TRY(hipMemcpyAsync(so->cells_trace, L.cells_trace, sizeof(int) * ntr,
+                               hipMemcpyDeviceToHost, s));
     if (fe) (void)hipEventRecord(fe[2], s);
     HIP_TRY(hipStreamSynchronize(s));
     const DevState fin = *ws->h_state;

@@ -39,7 +39,9 @@ class Stats(ctypes.Structure):
                 ("total_ms", ctypes.c_double), ("update_kernel_ms", ctypes.c_double),
                 ("update_launches", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("workspace_bytes", ctypes.c_int64), ("itr_ms", ctypes.POINTER(ctypes.c_float)),
-                ("itr_ms_cap", ctypes.c_int32), ("pad2", ctypes.c_int32)]
+                ("itr_ms_cap", ctypes.c_int32), ("trace_cap", ctypes.c_int32),
+                ("cells_trace", ctypes.POINTER(ctypes.c_int32)),
+                ("rc_trace", ctypes.POINTER(ctypes.c_double))]
 
 
 class EggSyncError(RuntimeError):

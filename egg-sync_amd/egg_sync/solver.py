@@ -41,6 +41,11 @@ def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-
     itr = np.zeros(itr_cap, np.float32)
     st.itr_ms = itr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
     st.itr_ms_cap = itr_cap
+    cells_tr = np.zeros(4096, np.int32)
+    rc_tr = np.zeros(4096, np.float64)
+    st.trace_cap = 4096
+    st.cells_trace = cells_tr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    st.rc_trace = rc_tr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     rc = nat.lib().egg_sync_solve_device(nat.ptr(D), n, d, float(eps), ctypes.byref(opt),
                                          nat.ptr(labels), nat.ptr(final), ctypes.byref(st),
                                          _stream())
@@ -50,6 +55,9 @@ def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-
                                          "workspace_bytes")}
     stats["stage_ms"] = list(st.stage_ms)
     stats["itr_ms"] = itr[: st.iterations].copy() if timing else None
+    it = min(st.iterations, 4096)
+    stats["cells_trace"] = cells_tr[:it].copy()
+    stats["rc_trace"] = rc_tr[:it].copy()
     return dict(labels=labels.cpu().numpy(), X=None if final is None else final.cpu().numpy(),
                 stats=stats, code=rc)
 

@@ -72,9 +72,11 @@ typedef struct egg_sync_stats {
     int32_t update_launches;
     int32_t pad;
     int64_t workspace_bytes; /* device bytes held for the solve ("space")          */
-    float *itr_ms;           /* optional caller array, per-iteration GPU time      */
+    float *itr_ms;           /* optional caller array, per-iteration GPU time (ms) */
     int32_t itr_ms_cap;      /* capacity of itr_ms                                 */
-    int32_t pad2;
+    int32_t trace_cap;       /* capacity of cells_trace / rc_trace                 */
+    int32_t *cells_trace;    /* optional: non-empty cells C of each iteration      */
+    double *rc_trace;        /* optional: r_c of each iteration (stop rule RC)     */
 } egg_sync_stats;
 
 /* Fills *opt with the defaults of EGG_SynC (stop_rule EGG, labels cells, lam 0.999). */

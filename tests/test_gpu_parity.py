@@ -137,7 +137,13 @@ def test_sync_mode_matches_reference_golden(S, O):
     assert O.same_partition(a["labels"], g["labels"])
     assert np.array_equal(a["labels"], g["labels"])  # same greedy numbering too
     assert np.abs(a["X"] - g["X_final"]).max() <= 1e-6
-    assert abs(a["stats"]["rc"] - float(g["r_trace"][-1])) <= 1e-6
+    # r_c of the last pass is taken on its pre-update positions (SynC.cpp:285):
+    # FP64 sums here vs the oracle's FP64 r_c of the same positions; the
+    # reference's own value is an FP32 running sum over n points
+    it = int(g["iterations"])
+    pre = O.sync_solve(g["X"], float(g["eps"]), g["lam"], max_iter=it - 1)["X"]
+    assert abs(a["stats"]["rc"] - O.rc(pre, float(g["eps"]))) <= 1e-7
+    assert abs(a["stats"]["rc"] - float(g["r_trace"][-1])) <= 2e-6 * np.sqrt(len(g["X"])) / 10
 
 
 def test_rc_matches_oracle(S, O):
@@ -207,7 +213,8 @@ def test_torch_extension_api(S, O):
 
 
 def test_full_size_config1_properties(S, O):
-    """n = 1,000,000, d = 3, eps = 0.02: convergence, determinism, and a
+    """n = 1,000,000, d = 3, eps = 0.02: the oracle's full-size solve (fixture
+    written by tests/golden/make_oracle_fixtures.py), determinism, and a
     bit-exact pass of the converged state against the oracle at full size."""
     from egg_sync import data
     X, eps = data.config1()
@@ -215,6 +222,11 @@ def test_full_size_config1_properties(S, O):
     a = S.solve(X, eps)
     assert a["stats"]["status"] == 0
     _say("config1 iterations", a["stats"]["iterations"], "clusters", a["stats"]["n_clusters"])
+    ref = golden("c1_full_oracle")
+    assert a["stats"]["iterations"] == int(ref["iterations"])
+    assert O.same_partition(a["labels"], ref["labels"])
+    assert np.abs(a["X"][ref["sample_idx"]] - ref["sample_X"]).max() <= 1e-5
+    assert np.abs(a["X"].astype(np.float64).sum(0) - ref["X_sum"]).max() <= 1e-5 * len(X)
     b = S.solve(X, eps)
     assert np.array_equal(a["X"], b["X"]) and np.array_equal(a["labels"], b["labels"])
     # labels are the cell rule applied to the final positions
