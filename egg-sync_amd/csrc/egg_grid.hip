@@ -211,12 +211,14 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
 __global__ __launch_bounds__(kBlock) void k_wi_count(const int *__restrict__ cell_start,
                                                      const DevState *st,
                                                      int *__restrict__ nchunk,
+                                                     int *__restrict__ nonuni,
                                                      DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int C = st->C;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
         const int cnt = cell_start[c + 1] - cell_start[c];
         nchunk[c] = (cnt + kSumChunk - 1) / kSumChunk;
+        nonuni[c] = 0;
     }
 }
 
@@ -230,7 +232,8 @@ __global__ __launch_bounds__(kBlock) void k_summary(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const int *__restrict__ wi_off,
     const int *__restrict__ nchunk, Geo g, const DevState *st, float *__restrict__ Ts,
-    float *__restrict__ Tc, float *__restrict__ partial, DevState *st_loop) {
+    float *__restrict__ Tc, float *__restrict__ partial, int *__restrict__ nonuni,
+    DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int n = g.n, C = st->C, W = st->W;
     const int lane = __lane_id();
@@ -248,6 +251,15 @@ __global__ __launch_bounds__(kBlock) void k_summary(
         const int j0 = c0 + k * kSumChunk;
         const int j1 = min(c1, j0 + kSumChunk);
         const int nch = nchunk[c];
+        // uniform-cell flag: does any point differ bitwise from the cell's first point?
+        bool differs = false;
+        for (int j = j0 + lane; j < j1; j += kWave) {
+#pragma unroll
+            for (int l = 0; l < D; l++)
+                differs |= __float_as_uint(Xs[(long long)l * n + j]) !=
+                           __float_as_uint(Xs[(long long)l * n + c0]);
+        }
+        if (__any(differs) && lane == 0) atomicOr(&nonuni[c], 1);
 #pragma unroll
         for (int l = 0; l < D; l++) {
             const float a = (float)cell_g[(long long)l * n + c] * g.cs;
@@ -338,7 +350,7 @@ __global__ void k_store_total(const int *__restrict__ excl, const int *__restric
 void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
     const int n = g.n;
     const int gb = grid_for(n);
-    k_wi_count<<<gb, kBlock, 0, s>>>(b.cell_start, st, b.nchunk, st_loop);
+    k_wi_count<<<gb, kBlock, 0, s>>>(b.cell_start, st, b.nchunk, b.nonuni, st_loop);
     scan_exclusive(b.nchunk, b.wi_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
     k_store_total<<<1, 1, 0, s>>>(b.wi_off, b.nchunk, st, st_loop);
     // work items <= C + n / kSumChunk <= n + n / kSumChunk
@@ -347,7 +359,8 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
 #define SUM_CASE(DD)                                                                         \
     case DD:                                                                                 \
         k_summary<DD><<<wb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.wi_off,          \
-                                            b.nchunk, g, st, b.Ts, b.Tc, b.partial, st_loop); \
+                                            b.nchunk, g, st, b.Ts, b.Tc, b.partial, b.nonuni, \
+                                            st_loop);                                         \
         k_summary_combine<DD><<<gb, kBlock, 0, s>>>(b.cell_start, b.cell_g, b.wi_off,        \
                                                     b.nchunk, g, st, b.Ts, b.Tc, b.partial,  \
                                                     st_loop);                                \

@@ -37,6 +37,7 @@ struct GridBufs {
     int *cell_outer;     // [n]
     int *ob, *oe;        // [O]
     int *nchunk, *wi_off;// [n]
+    int *nonuni;         // [n]   1 if a cell holds two bitwise-different points
     float *Ts, *Tc;      // [d][n]
     float *partial;      // [n + n/kSumChunk + 1][4*d]
 };

@@ -160,6 +160,7 @@ Layout carve(void *base, const Geo &g) {
     b.oe = c.take<int>((size_t)g.O);
     b.nchunk = c.take<int>(n);
     b.wi_off = c.take<int>(n);
+    b.nonuni = c.take<int>(n);
     b.Ts = c.take<float>(n * d);
     b.Tc = c.take<float>(n * d);
     b.partial = c.take<float>((n + n / kSumChunk + 1) * 4 * d);

@@ -98,7 +98,8 @@ __global__ __launch_bounds__(kBlock) void k_update_egg(
     int *__restrict__ perm_out, const int *__restrict__ point_cell,
     const int *__restrict__ cell_start, const int *__restrict__ cell_g,
     const float *__restrict__ Ts, const float *__restrict__ Tc, const int *__restrict__ ob,
-    const int *__restrict__ oe, Geo g, DevState *st, DebugOut dbg) {
+    const int *__restrict__ oe, const int *__restrict__ nonuni, Geo g, DevState *st,
+    DebugOut dbg) {
     if (st->done) return;
     const int n = g.n;
     const int R = g.R, od = g.od;
@@ -158,6 +159,10 @@ __global__ __launch_bounds__(kBlock) void k_update_egg(
                         if (border) {
                             if (c == c0) {
                                 half += s1 - s0;
+                            } else if (!nonuni[c]) {
+                                // bitwise-equal points: one test decides for all
+                                float xq[D];
+                                if (dist2_to<D>(x, Xs, n, s0, xq) <= thr_half) half += s1 - s0;
                             } else {
                                 for (int j = s0; j < s1; j++) {
                                     float xq[D];
@@ -166,12 +171,26 @@ __global__ __launch_bounds__(kBlock) void k_update_egg(
                             }
                         }
                     } else if (included) {
-                        for (int j = s0; j < s1; j++) {
+                        if (!nonuni[c]) {
+                            // bitwise-equal points: the same test and the same terms,
+                            // added one by one in the same order
                             float xq[D];
-                            if (dist2_to<D>(x, Xs, n, j, xq) <= thr) {
+                            if (dist2_to<D>(x, Xs, n, s0, xq) <= thr) {
 #pragma unroll
-                                for (int l = 0; l < D; l++) sml[l] += sinf(xq[l] - x[l]);
-                                N++;
+                                for (int l = 0; l < D; l++) {
+                                    const float t = sinf(xq[l] - x[l]);
+                                    for (int j = s0; j < s1; j++) sml[l] += t;
+                                }
+                                N += s1 - s0;
+                            }
+                        } else {
+                            for (int j = s0; j < s1; j++) {
+                                float xq[D];
+                                if (dist2_to<D>(x, Xs, n, j, xq) <= thr) {
+#pragma unroll
+                                    for (int l = 0; l < D; l++) sml[l] += sinf(xq[l] - x[l]);
+                                    N++;
+                                }
                             }
                         }
                     }
@@ -203,7 +222,7 @@ template <int D>
 __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__restrict__ Xs,
                            const int *__restrict__ cell_start, const int *__restrict__ cell_g,
                            const int *__restrict__ ob, const int *__restrict__ oe,
-                           const Geo &g) {
+                           const int *__restrict__ nonuni, const Geo &g) {
     const int n = g.n, od = g.od, R = g.R_half;
     float mbr[D];
     int g0[D];
@@ -235,7 +254,9 @@ __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__rest
                 float sm, lg;
                 box_dist<D>(gc, xq, g.cs, sm, lg);
                 if (!(sm <= g.thr_half)) continue;
-                for (int j = cell_start[c]; j < cell_start[c + 1]; j++) {
+                // one representative of a cell of bitwise-equal points
+                const int j_end = nonuni[c] ? cell_start[c + 1] : cell_start[c] + 1;
+                for (int j = cell_start[c]; j < j_end; j++) {
                     float x2[D];
                     if (dist2_to<D>(xq, Xs, n, j, x2) <= g.thr_half) {
 #pragma unroll
@@ -268,8 +289,9 @@ __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__rest
 // Runs only when every point is locally synced; any hit clears the flag.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_extra_check(
-    const float *__restrict__ Xs, const int *__restrict__ cell_start,
-    const int *__restrict__ cell_g, const int *__restrict__ ob, const int *__restrict__ oe,
+    const float *__restrict__ Xs, const int *__restrict__ point_cell,
+    const int *__restrict__ cell_start, const int *__restrict__ cell_g,
+    const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ nonuni,
     Geo g, DevState *st) {
     if (st->done || st->flag == 0) return;
     const int n = g.n, od = g.od, R = g.R_extra;
@@ -280,11 +302,19 @@ __global__ __launch_bounds__(kBlock) void k_extra_check(
         if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
         float xp[D];
         int g0[D];
+        // the result depends on x_p only: a point bitwise equal to its
+        // predecessor in the same cell repeats the predecessor's result
+        const int ci = point_cell[i];
+        bool same = i > cell_start[ci];
 #pragma unroll
         for (int l = 0; l < D; l++) {
             xp[l] = Xs[(long long)l * n + i];
             g0[l] = cell_coord(xp[l], g.cs, g.w);
+            if (same) same = __float_as_uint(xp[l]) == __float_as_uint(Xs[(long long)l * n + i - 1]);
         }
+        if (same) continue;
+        float lq[D];          // last partner whose MBR was tested (the MBR depends on x_q only)
+        bool have_lq = false;
         const int g0od = select_dim<D>(g0, od);
         Odo<D> odo;
         odo.init(od, R);
@@ -309,11 +339,20 @@ __global__ __launch_bounds__(kBlock) void k_extra_check(
                     const bool included = sm <= g.thr_extra;
                     const bool full = lg <= g.thr_eps;
                     if (full || !included) continue;
-                    for (int j = cell_start[c]; j < cell_start[c + 1]; j++) {
+                    const int j_end = nonuni[c] ? cell_start[c + 1] : cell_start[c] + 1;
+                    for (int j = cell_start[c]; j < j_end; j++) {
                         float xq[D];
                         const float s = dist2_to<D>(xp, Xs, n, j, xq);
                         if (!(s <= g.thr_eps) && s <= g.thr_extra) {
-                            if (mbr_in_eps<D>(xp, xq, Xs, cell_start, cell_g, ob, oe, g)) {
+                            bool rep = have_lq;
+#pragma unroll
+                            for (int l = 0; l < D; l++)
+                                rep = rep && __float_as_uint(lq[l]) == __float_as_uint(xq[l]);
+                            if (rep) continue;
+#pragma unroll
+                            for (int l = 0; l < D; l++) lq[l] = xq[l];
+                            have_lq = true;
+                            if (mbr_in_eps<D>(xp, xq, Xs, cell_start, cell_g, ob, oe, nonuni, g)) {
                                 hit = true;
                                 break;
                             }
@@ -466,7 +505,7 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
     case DD:                                                                                  \
         k_update_egg<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.X, b.perm_s, b.perm, b.point_cell,     \
                                                b.cell_start, b.cell_g, b.Ts, b.Tc, b.ob,      \
-                                               b.oe, g, st, d);                               \
+                                               b.oe, b.nonuni, g, st, d);                     \
         break;
     switch (g.d) {
         UPD_CASE(1) UPD_CASE(2) UPD_CASE(3) UPD_CASE(4)
@@ -479,8 +518,8 @@ void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s) 
     const int gb = grid_for(g.n);
 #define EX_CASE(DD)                                                                       \
     case DD:                                                                              \
-        k_extra_check<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.ob, b.oe, \
-                                                g, st);                                   \
+        k_extra_check<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.point_cell, b.cell_start, b.cell_g, \
+                                                b.ob, b.oe, b.nonuni, g, st);               \
         break;
     switch (g.d) {
         EX_CASE(1) EX_CASE(2) EX_CASE(3) EX_CASE(4)

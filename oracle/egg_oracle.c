@@ -223,6 +223,7 @@ int eo_extra_check_f32(const float *X, int64_t n, int32_t d, float eps) {
     geo_f32_from(&g, &geo);
     grid_t_f32 G;
     if (build_grid_f32(&geo, X, NULL, &G) != 0) return -2;
+    cell_summaries_f32(&geo, X, &G);
     int r = extra_check_f32(&geo, X, &G);
     grid_free_f32(&G);
     return r;
