@@ -63,7 +63,7 @@ struct DevState {
     int stop_rule;     // 0: EGG local sync + extra check; 1: r_c >= lam
     int n_clusters;
     float lam;
-    int pad;
+    int pair_count;    // extra check: (p, q) pairs found this iteration
     double rc;         // order parameter of the last r_c evaluation
 };
 

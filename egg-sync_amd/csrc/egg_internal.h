@@ -40,6 +40,8 @@ struct GridBufs {
     int *nonuni;         // [n]   1 if a cell holds two bitwise-different points
     float *Ts, *Tc;      // [d][n]
     float *partial;      // [n + n/kSumChunk + 1][4*d]
+    int *pairs;          // [2 * pair_cap] extra-check (p, q) sorted positions
+    int pair_cap;
 };
 
 void launch_aos_to_soa(const float *aos, float *soa, int *perm, int n, int d, hipStream_t s);

@@ -164,6 +164,8 @@ Layout carve(void *base, const Geo &g) {
     b.Ts = c.take<float>(n * d);
     b.Tc = c.take<float>(n * d);
     b.partial = c.take<float>((n + n / kSumChunk + 1) * 4 * d);
+    b.pair_cap = (int)std::max<size_t>(n, 65536);
+    b.pairs = c.take<int>(2 * (size_t)b.pair_cap);
     L.st = c.take<DevState>(1);
     L.labels = c.take<int>(n);
     L.io = c.take<float>(n * d);

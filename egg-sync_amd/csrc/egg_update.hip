@@ -283,16 +283,19 @@ __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__rest
     return dd <= g.thr_eps;
 }
 
-// kernel_float_grid_extra_check_part1 + part2 (EGG_SynC.cu:767-1034,1162-1203),
-// fused: each point looks for partners q with eps < |p-q| <= eps_extra in
-// cells that are not fully within eps, and tests the MBR of (p, q) at once.
+// kernel_float_grid_extra_check_part1 (EGG_SynC.cu:900-1034): each point
+// looks for partners q with eps < |p-q| <= eps_extra in cells that are not
+// fully within eps and queues the pair for k_extra_mbr.  Pairs are
+// deduplicated by position (the MBR test depends on x_p and x_q only): a point
+// equal to its predecessor is skipped, and so is a partner equal to the last
+// one queued.  Should the queue overflow, the thread tests the pair itself.
 // Runs only when every point is locally synced; any hit clears the flag.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_extra_check(
     const float *__restrict__ Xs, const int *__restrict__ point_cell,
     const int *__restrict__ cell_start, const int *__restrict__ cell_g,
     const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ nonuni,
-    Geo g, DevState *st) {
+    int *__restrict__ pairs, int pair_cap, Geo g, DevState *st) {
     if (st->done || st->flag == 0) return;
     const int n = g.n, od = g.od, R = g.R_extra;
     const int *cg_od = cell_g + (long long)od * n;
@@ -352,7 +355,12 @@ __global__ __launch_bounds__(kBlock) void k_extra_check(
 #pragma unroll
                             for (int l = 0; l < D; l++) lq[l] = xq[l];
                             have_lq = true;
-                            if (mbr_in_eps<D>(xp, xq, Xs, cell_start, cell_g, ob, oe, nonuni, g)) {
+                            const int slot = atomicAdd(&st->pair_count, 1);
+                            if (slot < pair_cap) {
+                                pairs[2 * slot] = i;
+                                pairs[2 * slot + 1] = j;
+                            } else if (mbr_in_eps<D>(xp, xq, Xs, cell_start, cell_g, ob, oe,
+                                                     nonuni, g)) {
                                 hit = true;
                                 break;
                             }
@@ -470,9 +478,91 @@ __global__ __launch_bounds__(kBlock) void k_rc_finish(const double *__restrict__
     if (threadIdx.x == 0) st->rc = s_r[0] / (double)n;
 }
 
+// kernel_float_grid_extra_check_part2 + MBR_in_eps (EGG_SynC.cu:1037-1203),
+// one wavefront per queued pair: the lanes share the scan of q's
+// eps/2-neighbourhood and keep, per dimension, the smallest squared distance
+// to p of a candidate coordinate (the reference keeps the coordinate with the
+// smallest one, so its MBR distance is the sum of these minima); a lane-min
+// reduction then gives the same value as the sequential scan.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_extra_mbr(
+    const float *__restrict__ Xs, const int *__restrict__ cell_start,
+    const int *__restrict__ cell_g, const int *__restrict__ ob, const int *__restrict__ oe,
+    const int *__restrict__ nonuni, const int *__restrict__ pairs, int pair_cap, Geo g,
+    DevState *st) {
+    if (st->done || st->flag == 0) return;
+    const int n = g.n, od = g.od, R = g.R_half;
+    const int npairs = min(st->pair_count, pair_cap);
+    const int lane = __lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    const int *cg_od = cell_g + (long long)od * n;
+    for (int k = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; k < npairs;
+         k += waves) {
+        if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+        const int ip = pairs[2 * k], iq = pairs[2 * k + 1];
+        float xp[D], xq[D], best[D];
+        int g0[D];
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            xp[l] = Xs[(long long)l * n + ip];
+            xq[l] = Xs[(long long)l * n + iq];
+            g0[l] = cell_coord(xq[l], g.cs, g.w);
+            float t = xq[l] - xp[l];  // the MBR starts at q
+            t *= t;
+            best[l] = t;
+        }
+        const int g0od = select_dim<D>(g0, od);
+        Odo<D> odo;
+        odo.init(od, R);
+        while (!odo.finished) {
+            const int o = odo.outer(g0, g);
+            if (o >= 0) {
+                int cb = ob[o];
+                const int ce = oe[o];
+                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+                for (int c = cb; c < ce; c++) {
+                    int gc[D];
+#pragma unroll
+                    for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
+                    if (od < D && select_dim<D>(gc, od) > g0od + R + 1) break;
+                    bool skip = false;
+#pragma unroll
+                    for (int l = 0; l < D; l++)
+                        if (l > od && (gc[l] - g0[l] > R + 1 || g0[l] - gc[l] > R + 1)) skip = true;
+                    if (skip) continue;
+                    float sm, lg;
+                    box_dist<D>(gc, xq, g.cs, sm, lg);
+                    if (!(sm <= g.thr_half)) continue;
+                    const int j_end = nonuni[c] ? cell_start[c + 1] : cell_start[c] + 1;
+                    for (int j = cell_start[c] + lane; j < j_end; j += kWave) {
+                        float x2[D];
+                        if (dist2_to<D>(xq, Xs, n, j, x2) <= g.thr_half) {
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                float nd = x2[l] - xp[l];
+                                nd *= nd;
+                                best[l] = fminf(best[l], nd);
+                            }
+                        }
+                    }
+                }
+            }
+            odo.next(od, R);
+        }
+#pragma unroll
+        for (int l = 0; l < D; l++)
+            for (int o = kWave / 2; o > 0; o >>= 1) best[l] = fminf(best[l], __shfl_xor(best[l], o, kWave));
+        float dd = 0.f;
+#pragma unroll
+        for (int l = 0; l < D; l++) dd += best[l];
+        if (dd <= g.thr_eps && lane == 0) atomicAnd(&st->flag, 0);
+    }
+}
+
 __global__ void k_set_flag(DevState *st) {
     if (st->done) return;
     st->flag = 1;
+    st->pair_count = 0;
 }
 
 // End of an iteration (EGG_SynC.cu:1322,1425 / SynC.cpp:277).
@@ -519,7 +609,10 @@ void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s) 
 #define EX_CASE(DD)                                                                       \
     case DD:                                                                              \
         k_extra_check<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.point_cell, b.cell_start, b.cell_g, \
-                                                b.ob, b.oe, b.nonuni, g, st);               \
+                                                b.ob, b.oe, b.nonuni, b.pairs, b.pair_cap,  \
+                                                g, st);                                     \
+        k_extra_mbr<DD><<<1024, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.ob, b.oe,   \
+                                                b.nonuni, b.pairs, b.pair_cap, g, st);      \
         break;
     switch (g.d) {
         EX_CASE(1) EX_CASE(2) EX_CASE(3) EX_CASE(4)
