@@ -26,6 +26,7 @@ constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kSortItems = 8;     // items per thread in radix passes
 constexpr int kSortTile = kBlock * kSortItems;
 constexpr int kSumChunk = 2048;   // max points per summary work item
+constexpr int kTile = 64;         // max points per update tile (one wavefront)
 
 // Grid geometry and thresholds, computed on the host (EGG_SynC.cu:1249-1260).
 struct Geo {
@@ -64,6 +65,8 @@ struct DevState {
     int n_clusters;
     float lam;
     int pair_count;    // extra check: (p, q) pairs found this iteration
+    int T;             // update tiles (<= 64 points of one cell each)
+    int tile_next;     // work-queue head of the update kernel
     double rc;         // order parameter of the last r_c evaluation
 };
 

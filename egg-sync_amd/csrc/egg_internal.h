@@ -38,6 +38,7 @@ struct GridBufs {
     int *ob, *oe;        // [O]
     int *nchunk, *wi_off;// [n]
     int *nonuni;         // [n]   1 if a cell holds two bitwise-different points
+    int *ntile, *tile_off;// [n]  update tiles per cell, exclusive scan
     float *Ts, *Tc;      // [d][n]
     float *partial;      // [n + n/kSumChunk + 1][4*d]
     int *pairs;          // [2 * pair_cap] extra-check (p, q) sorted positions

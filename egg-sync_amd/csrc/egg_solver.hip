@@ -161,6 +161,8 @@ Layout carve(void *base, const Geo &g) {
     b.nchunk = c.take<int>(n);
     b.wi_off = c.take<int>(n);
     b.nonuni = c.take<int>(n);
+    b.ntile = c.take<int>(n);
+    b.tile_off = c.take<int>(n);
     b.Ts = c.take<float>(n * d);
     b.Tc = c.take<float>(n * d);
     b.partial = c.take<float>((n + n / kSumChunk + 1) * 4 * d);

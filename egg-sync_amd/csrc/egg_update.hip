@@ -8,6 +8,8 @@
 // in the remaining dimensions to |dg| <= R+1 by a binary search over the
 // column's cells (sorted by g_od) -- cells farther than that fail the
 // reference's box test by more than a cell side, so results do not change.
+#include <algorithm>
+
 #include "egg_internal.h"
 
 namespace egg {
@@ -91,38 +93,84 @@ __device__ __forceinline__ float dist2_to(const float *x, const float *__restric
     return s;
 }
 
-// kernel_itr_grid_STAD5_5_1 (EGG_SynC.cu:551-763).
+// A source point of a partial cell staged in LDS: its coordinates and the
+// sine / cosine of its offset from the tile's cell corner.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_update_egg(
+struct SrcPt {
+    static constexpr int P = (D + 3) / 4 * 4;
+    float x[P], sv[P], cv[P];
+};
+
+// Makes the LDS writes of every lane visible to the reads of the others
+// (wavefront scope; the LDS serves one wavefront's accesses in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// kernel_itr_grid_STAD5_5_1 (EGG_SynC.cu:551-763), one wavefront per tile of
+// <= 64 points of one cell, tiles taken from a work queue.  All lanes share
+// the cell, so the walk over candidate cells and every cell load are
+// wave-uniform; each lane runs the reference's box test on its own point, so
+// full / partial / out, N, half and the sync flag are exactly the reference's.
+// Full cells use the per-cell sin/cos totals (EGG_SynC.cu:648-663).  Partial
+// cells are streamed through LDS in chunks of 64 points: each lane loads one
+// point and computes sin / cos of v = x_q - a0 (a0 = the tile's cell corner),
+// then every lane reads every point as a broadcast and, for points within
+// eps, adds sin(x_q - x) = sin(v) cos(u) - cos(v) sin(u) with u = x - a0.
+// Both angles are below eps + 2 cell sides, so the identity keeps the error
+// of each term at a few ulps of eps, like a direct sinf(x_q - x).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_update_tile(
     const float *__restrict__ Xs, float *__restrict__ Xout, const int *__restrict__ perm_s,
-    int *__restrict__ perm_out, const int *__restrict__ point_cell,
-    const int *__restrict__ cell_start, const int *__restrict__ cell_g,
-    const float *__restrict__ Ts, const float *__restrict__ Tc, const int *__restrict__ ob,
-    const int *__restrict__ oe, const int *__restrict__ nonuni, Geo g, DevState *st,
-    DebugOut dbg) {
+    int *__restrict__ perm_out, const int *__restrict__ cell_start,
+    const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
+    const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ nonuni,
+    const int *__restrict__ tile_off, Geo g, DevState *st, DebugOut dbg) {
     if (st->done) return;
-    const int n = g.n;
-    const int R = g.R, od = g.od;
+    __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
+    const int n = g.n, R = g.R, od = g.od;
+    const int C = st->C, T = st->T;
     const float thr = g.thr_eps, thr_half = g.thr_half;
+    const int lane = __lane_id();
+    SrcPt<D> *src = s_src[threadIdx.x / kWave];
+    const int *cg_od = cell_g + (long long)od * n;
     bool unsynced = false;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float x[D], sx[D], cx[D], sum[D], sml[D];
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&st->tile_next, 1);
+        t = __shfl(t, 0, kWave);
+        if (t >= T) break;
+        int lo = 0, hi = C - 1;  // cell of tile t: largest c with tile_off[c] <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile_off[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int c0 = lo;
+        const int s0c = cell_start[c0], s1c = cell_start[c0 + 1];
+        const int own = s1c - s0c;
+        const int i = s0c + (t - tile_off[c0]) * kTile + lane;
+        const bool active = i < s1c;
+        float x[D], a0[D], sx[D], cx[D], su[D], cu[D], sum[D], sml[D];
         int g0[D];
-        const int c0 = point_cell[i];
-        const int own = cell_start[c0 + 1] - cell_start[c0];
         bool border = false;
 #pragma unroll
         for (int l = 0; l < D; l++) {
-            x[l] = Xs[(long long)l * n + i];
             g0[l] = cell_g[(long long)l * n + c0];
+            a0[l] = (float)g0[l] * g.cs;
+            x[l] = active ? Xs[(long long)l * n + i] : a0[l];
             sum[l] = 0.f;
             sml[l] = 0.f;
             sx[l] = sinf(x[l]);
             cx[l] = cosf(x[l]);
-            border |= on_border(x[l], g.cs);
+            const float u = x[l] - a0[l];
+            su[l] = sinf(u);
+            cu[l] = cosf(u);
+            border |= active && on_border(x[l], g.cs);
         }
         const int g0od = select_dim<D>(g0, od);
-        const int *cg_od = cell_g + (long long)od * n;
         int N = 0, half = 0;
         Odo<D> odo;
         odo.init(od, R);
@@ -144,16 +192,16 @@ __global__ __launch_bounds__(kBlock) void k_update_egg(
                     if (skip) continue;
                     float sm, lg;
                     box_dist<D>(gc, x, g.cs, sm, lg);
-                    const bool included = sm <= thr;
-                    const bool full = lg <= thr;
+                    const bool full = active && lg <= thr;
+                    const bool part = active && !full && sm <= thr;
                     const int s0 = cell_start[c], s1 = cell_start[c + 1];
                     if (full) {
 #pragma unroll
                         for (int l = 0; l < D; l++) {
                             const float a = Ts[(long long)l * n + c] * cx[l];
                             const float b = Tc[(long long)l * n + c] * sx[l];
-                            const float t = a - b;
-                            sum[l] += t;
+                            const float tt = a - b;
+                            sum[l] += tt;
                         }
                         N += s1 - s0;
                         if (border) {
@@ -170,49 +218,84 @@ __global__ __launch_bounds__(kBlock) void k_update_egg(
                                 }
                             }
                         }
-                    } else if (included) {
-                        if (!nonuni[c]) {
-                            // bitwise-equal points: the same test and the same terms,
-                            // added one by one in the same order
-                            float xq[D];
-                            if (dist2_to<D>(x, Xs, n, s0, xq) <= thr) {
+                    }
+                    if (!__any(part)) continue;
+                    if (!nonuni[c]) {
+                        // bitwise-equal points: the same test and the same term,
+                        // added once per point in order
+                        float xq[D];
+                        const float s = dist2_to<D>(x, Xs, n, s0, xq);
+                        if (part && s <= thr) {
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float v = xq[l] - a0[l];
+                                const float p1 = sinf(v) * cu[l];
+                                const float p2 = cosf(v) * su[l];
+                                const float tt = p1 - p2;
+                                for (int j = s0; j < s1; j++) sml[l] += tt;
+                            }
+                            N += s1 - s0;
+                        }
+                        continue;
+                    }
+                    for (int base = s0; base < s1; base += kWave) {
+                        const int j = base + lane;
+                        if (j < s1) {
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float xq = Xs[(long long)l * n + j];
+                                const float v = xq - a0[l];
+                                src[lane].x[l] = xq;
+                                src[lane].sv[l] = sinf(v);
+                                src[lane].cv[l] = cosf(v);
+                            }
+                        }
+                        wave_sync();
+                        const int m = min(kWave, s1 - base);
+                        if (part) {
+#pragma unroll 4
+                            for (int k = 0; k < m; k++) {
+                                const SrcPt<D> &q = src[k];
+                                float s = 0.f;
 #pragma unroll
                                 for (int l = 0; l < D; l++) {
-                                    const float t = sinf(xq[l] - x[l]);
-                                    for (int j = s0; j < s1; j++) sml[l] += t;
+                                    const float df = x[l] - q.x[l];
+                                    s += df * df;
                                 }
-                                N += s1 - s0;
-                            }
-                        } else {
-                            for (int j = s0; j < s1; j++) {
-                                float xq[D];
-                                if (dist2_to<D>(x, Xs, n, j, xq) <= thr) {
+                                if (s <= thr) {
 #pragma unroll
-                                    for (int l = 0; l < D; l++) sml[l] += sinf(xq[l] - x[l]);
+                                    for (int l = 0; l < D; l++) {
+                                        const float p1 = q.sv[l] * cu[l];
+                                        const float p2 = q.cv[l] * su[l];
+                                        sml[l] += p1 - p2;
+                                    }
                                     N++;
                                 }
                             }
                         }
+                        wave_sync();
                     }
                 }
             }
             odo.next(od, R);
         }
+        if (active) {
 #pragma unroll
-        for (int l = 0; l < D; l++) {
-            float t = sum[l] + sml[l];
-            t = t / (float)N;
-            Xout[(long long)l * n + i] = x[l] + t;
+            for (int l = 0; l < D; l++) {
+                float tt = sum[l] + sml[l];
+                tt = tt / (float)N;
+                Xout[(long long)l * n + i] = x[l] + tt;
+            }
+            perm_out[i] = perm_s[i];
+            const bool synced = !((!border && N != own) || (border && N != half));
+            if (!synced) unsynced = true;
+            if (dbg.N) dbg.N[i] = N;
+            if (dbg.half) dbg.half[i] = half;
+            if (dbg.border) dbg.border[i] = border ? 1 : 0;
+            if (dbg.synced) dbg.synced[i] = synced ? 1 : 0;
         }
-        perm_out[i] = perm_s[i];
-        const bool synced = !((!border && N != own) || (border && N != half));
-        if (!synced) unsynced = true;
-        if (dbg.N) dbg.N[i] = N;
-        if (dbg.half) dbg.half[i] = half;
-        if (dbg.border) dbg.border[i] = border ? 1 : 0;
-        if (dbg.synced) dbg.synced[i] = synced ? 1 : 0;
     }
-    if (__any(unsynced) && __lane_id() == 0) atomicAnd(&st->flag, 0);
+    if (__any(unsynced) && lane == 0) atomicAnd(&st->flag, 0);
 }
 
 // MBR_in_eps (EGG_SynC.cu:1037-1159) for the pair (p, q): per dimension the
@@ -563,6 +646,7 @@ __global__ void k_set_flag(DevState *st) {
     if (st->done) return;
     st->flag = 1;
     st->pair_count = 0;
+    st->tile_next = 0;
 }
 
 // End of an iteration (EGG_SynC.cu:1322,1425 / SynC.cpp:277).
@@ -587,15 +671,19 @@ static inline int grid_for(long long n) {
     return (int)b;
 }
 
+// Enough workgroups to fill the chip; they drain the tile queue and exit.
+constexpr int kUpdateBlocks = 2048;
+
 void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *dbg,
                        hipStream_t s) {
     DebugOut d = dbg ? *dbg : DebugOut{nullptr, nullptr, nullptr, nullptr};
-    const int gb = grid_for(g.n);
+    const long long tiles_max = (long long)g.n / kTile + g.n;  // <= n/64 + C
+    const int gb = (int)std::min<long long>(kUpdateBlocks, (tiles_max + 3) / 4);
 #define UPD_CASE(DD)                                                                          \
     case DD:                                                                                  \
-        k_update_egg<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.X, b.perm_s, b.perm, b.point_cell,     \
-                                               b.cell_start, b.cell_g, b.Ts, b.Tc, b.ob,      \
-                                               b.oe, b.nonuni, g, st, d);                     \
+        k_update_tile<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.X, b.perm_s, b.perm, b.cell_start,    \
+                                                b.cell_g, b.Ts, b.Tc, b.ob, b.oe, b.nonuni,   \
+                                                b.tile_off, g, st, d);                        \
         break;
     switch (g.d) {
         UPD_CASE(1) UPD_CASE(2) UPD_CASE(3) UPD_CASE(4)
