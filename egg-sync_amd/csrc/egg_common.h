@@ -67,6 +67,8 @@ struct DevState {
     int pair_count;    // extra check: (p, q) pairs found this iteration
     int T;             // update tiles (<= 64 points of one cell each)
     int tile_next;     // work-queue head of the update kernel
+    int xtile_next;    // work-queue head of the extra check
+    int pad;
     double rc;         // order parameter of the last r_c evaluation
 };
 

@@ -212,15 +212,28 @@ __global__ __launch_bounds__(kBlock) void k_wi_count(const int *__restrict__ cel
                                                      const DevState *st,
                                                      int *__restrict__ nchunk,
                                                      int *__restrict__ nonuni,
-                                                     int *__restrict__ ntile,
                                                      DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int C = st->C;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
         const int cnt = cell_start[c + 1] - cell_start[c];
         nchunk[c] = (cnt + kSumChunk - 1) / kSumChunk;
-        ntile[c] = (cnt + kTile - 1) / kTile;
         nonuni[c] = 0;
+    }
+}
+
+// Update tiles per cell: ceil(cnt / 64), or one for a cell of equal points
+// (all its points get the same result).
+__global__ __launch_bounds__(kBlock) void k_tile_count(const int *__restrict__ cell_start,
+                                                       const int *__restrict__ nonuni,
+                                                       const DevState *st,
+                                                       int *__restrict__ ntile,
+                                                       DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int C = st->C;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+        const int cnt = cell_start[c + 1] - cell_start[c];
+        ntile[c] = nonuni[c] ? (cnt + kTile - 1) / kTile : 1;
     }
 }
 
@@ -342,23 +355,20 @@ __global__ __launch_bounds__(kBlock) void k_summary_combine(
     }
 }
 
-// Totals of the two per-cell exclusive scans: summary work items and update tiles.
-__global__ void k_store_totals(const int *__restrict__ wi_off, const int *__restrict__ nchunk,
-                               const int *__restrict__ tile_off, const int *__restrict__ ntile,
-                               DevState *st, DevState *st_loop) {
+// Total of a per-cell exclusive scan (summary work items or update tiles).
+__global__ void k_store_total(const int *__restrict__ off, const int *__restrict__ cnt,
+                              DevState *st, int *out, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int C = st->C;
-    st->W = C > 0 ? wi_off[C - 1] + nchunk[C - 1] : 0;
-    st->T = C > 0 ? tile_off[C - 1] + ntile[C - 1] : 0;
+    *out = C > 0 ? off[C - 1] + cnt[C - 1] : 0;
 }
 
 void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
     const int n = g.n;
     const int gb = grid_for(n);
-    k_wi_count<<<gb, kBlock, 0, s>>>(b.cell_start, st, b.nchunk, b.nonuni, b.ntile, st_loop);
+    k_wi_count<<<gb, kBlock, 0, s>>>(b.cell_start, st, b.nchunk, b.nonuni, st_loop);
     scan_exclusive(b.nchunk, b.wi_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
-    scan_exclusive(b.ntile, b.tile_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
-    k_store_totals<<<1, 1, 0, s>>>(b.wi_off, b.nchunk, b.tile_off, b.ntile, st, st_loop);
+    k_store_total<<<1, 1, 0, s>>>(b.wi_off, b.nchunk, st, &st->W, st_loop);
     // work items <= C + n / kSumChunk <= n + n / kSumChunk
     const long long max_wi = (long long)n + n / kSumChunk + 1;
     const int wb = grid_for(max_wi * kWave, kBlock, 16384);
@@ -376,6 +386,9 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
         SUM_CASE(5) SUM_CASE(6) SUM_CASE(7) SUM_CASE(8)
     }
 #undef SUM_CASE
+    k_tile_count<<<gb, kBlock, 0, s>>>(b.cell_start, b.nonuni, st, b.ntile, st_loop);
+    scan_exclusive(b.ntile, b.tile_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
+    k_store_total<<<1, 1, 0, s>>>(b.tile_off, b.ntile, st, &st->T, st_loop);
 }
 
 }  // namespace egg

@@ -39,6 +39,8 @@ struct GridBufs {
     int *nchunk, *wi_off;// [n]
     int *nonuni;         // [n]   1 if a cell holds two bitwise-different points
     int *ntile, *tile_off;// [n]  update tiles per cell, exclusive scan
+    float *cres;         // [d][n] result of a single-tile cell of equal points
+    int *cdbg;           // [4n]  its N / half / border / synced
     float *Ts, *Tc;      // [d][n]
     float *partial;      // [n + n/kSumChunk + 1][4*d]
     int *pairs;          // [2 * pair_cap] extra-check (p, q) sorted positions

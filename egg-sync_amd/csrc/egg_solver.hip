@@ -163,6 +163,8 @@ Layout carve(void *base, const Geo &g) {
     b.nonuni = c.take<int>(n);
     b.ntile = c.take<int>(n);
     b.tile_off = c.take<int>(n);
+    b.cres = c.take<float>(n * d);
+    b.cdbg = c.take<int>(4 * n);
     b.Ts = c.take<float>(n * d);
     b.Tc = c.take<float>(n * d);
     b.partial = c.take<float>((n + n / kSumChunk + 1) * 4 * d);

@@ -1,13 +1,16 @@
 // Neighbour update, local-sync flag, extra check and FP64 order parameter
 // (EGG_SynC.cu:551-763, 767-1203; SynC.cpp:30-56), gfx950.
 //
-// One thread per point in cell-sorted order, so the lanes of a wavefront
-// mostly share one cell and walk the same candidate cells: cell metadata and
-// neighbour coordinates are wave-uniform loads served by the cache.  Candidate
-// cells come from the outer-cell box of radius R (as the reference), pruned
-// in the remaining dimensions to |dg| <= R+1 by a binary search over the
-// column's cells (sorted by g_od) -- cells farther than that fail the
-// reference's box test by more than a cell side, so results do not change.
+// The EGG kernels work on tiles: one wavefront per tile of <= 64 points of one
+// cell, tiles taken from a work queue.  All lanes of a tile share the cell, so
+// the walk over candidate cells and every cell load are wave-uniform while
+// each lane runs the reference's per-point tests.  Candidate cells come from
+// the outer-cell box of radius R (as the reference), pruned in the remaining
+// dimensions to |dg| <= R+1 by a binary search over the column's cells (sorted
+// by g_od): cells farther than that fail the reference's box test by more than
+// a cell side, so results do not change.  Cells whose points are all bitwise
+// equal form a single tile: every point of such a cell has the same inputs and
+// so the same result, which a scatter pass copies to the rest of the cell.
 #include <algorithm>
 
 #include "egg_internal.h"
@@ -15,8 +18,7 @@
 namespace egg {
 
 // Odometer over the outer-cell offsets [-R, R]^od, dim 0 fastest (the
-// enumeration order of EGG_SynC.cu:407-451).  Unrolled over D so the offsets
-// stay in registers.
+// enumeration order of EGG_SynC.cu:407-451), for per-thread walks.
 template <int D>
 struct Odo {
     int idx[D];
@@ -57,9 +59,58 @@ struct Odo {
     }
 };
 
-// Cells of outer cell o whose first remaining coordinate lies within
-// [g0od - rad, g0od + rad]: returns the first such cell (cells of a column are
-// sorted by g_od); the caller stops at the first cell above the range.
+// Walks the non-empty outer cells of the box [g0 - R, g0 + R]^od in the same
+// order as Odo, cooperatively: the 2R+1 outer cells of a row along dim 0 have
+// adjacent ids, so lane k loads the cell range of offset k and the wavefront
+// visits the non-empty ones in lane order.  f(cb, ce) receives each cell
+// range.  Called by a whole wavefront with wave-uniform g0 and R (2R+1 <= 64).
+template <int D, typename Fn>
+__device__ __forceinline__ void walk_columns(const int *g0, int R, const Geo &g,
+                                             const int *__restrict__ ob,
+                                             const int *__restrict__ oe, Fn &&f) {
+    const int od = g.od, lane = __lane_id();
+    int idx[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) idx[l] = -R;
+    for (;;) {
+        bool ok = true;
+        int base = 0;
+#pragma unroll
+        for (int l = 1; l < D; l++) {
+            if (l < od) {
+                const int v = g0[l] + idx[l];
+                if (v < 0 || v >= g.w) ok = false;
+                base += v * g.wpow[l];
+            }
+        }
+        if (ok) {
+            int cb = 0, ce = 0;
+            const int v0 = g0[0] - R + lane;
+            if (lane <= 2 * R && v0 >= 0 && v0 < g.w) {
+                cb = ob[base + v0];
+                ce = oe[base + v0];
+            }
+            unsigned long long m = __ballot(ce > cb);
+            while (m) {
+                const int k = __ffsll((long long)m) - 1;
+                m &= m - 1;
+                f(__builtin_amdgcn_readlane(cb, k), __builtin_amdgcn_readlane(ce, k));
+            }
+        }
+        bool carry = true;
+#pragma unroll
+        for (int l = 1; l < D; l++) {
+            if (l < od && carry) {
+                if (++idx[l] > R) idx[l] = -R;
+                else carry = false;
+            }
+        }
+        if (carry) break;
+    }
+}
+
+// First cell of [cb, ce) whose g_od is >= lo_v (cells of a column are sorted
+// by g_od); the caller stops at the first cell above its range.
 __device__ __forceinline__ int column_lower(const int *__restrict__ cell_g_od, int cb, int ce,
                                             int lo_v) {
     int lo = cb, hi = ce;
@@ -93,8 +144,25 @@ __device__ __forceinline__ float dist2_to(const float *x, const float *__restric
     return s;
 }
 
-// A source point of a partial cell staged in LDS: its coordinates and the
-// sine / cosine of its offset from the tile's cell corner.
+// Loads the D coordinates of cell c, returns false when the cell lies outside
+// the pruning window of the remaining dimensions (*stop: past the window in
+// g_od, so later cells of the column are outside too).
+template <int D>
+__device__ __forceinline__ bool cell_in_window(const int *__restrict__ cell_g, int n, int c,
+                                               const int *g0, int od, int rad, int g0od,
+                                               int *gc, bool *stop) {
+#pragma unroll
+    for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
+    *stop = od < D && select_dim<D>(gc, od) > g0od + rad;
+    bool in = true;
+#pragma unroll
+    for (int l = 0; l < D; l++)
+        if (l > od && (gc[l] - g0[l] > rad || g0[l] - gc[l] > rad)) in = false;
+    return in && !*stop;
+}
+
+// A source point of a cell staged in LDS: its coordinates and the sine /
+// cosine of its offset from the tile's cell corner.
 template <int D>
 struct SrcPt {
     static constexpr int P = (D + 3) / 4 * 4;
@@ -109,25 +177,39 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// kernel_itr_grid_STAD5_5_1 (EGG_SynC.cu:551-763), one wavefront per tile of
-// <= 64 points of one cell, tiles taken from a work queue.  All lanes share
-// the cell, so the walk over candidate cells and every cell load are
-// wave-uniform; each lane runs the reference's box test on its own point, so
-// full / partial / out, N, half and the sync flag are exactly the reference's.
-// Full cells use the per-cell sin/cos totals (EGG_SynC.cu:648-663).  Partial
-// cells are streamed through LDS in chunks of 64 points: each lane loads one
-// point and computes sin / cos of v = x_q - a0 (a0 = the tile's cell corner),
-// then every lane reads every point as a broadcast and, for points within
-// eps, adds sin(x_q - x) = sin(v) cos(u) - cos(v) sin(u) with u = x - a0.
-// Both angles are below eps + 2 cell sides, so the identity keeps the error
-// of each term at a few ulps of eps, like a direct sinf(x_q - x).
+// Cell of tile t: the largest c with tile_off[c] <= t.
+__device__ __forceinline__ int tile_cell(const int *__restrict__ tile_off, int C, int t) {
+    int lo = 0, hi = C - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_off[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int next_tile(int *counter) {
+    int t = 0;
+    if (__lane_id() == 0) t = atomicAdd(counter, 1);
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
+// kernel_itr_grid_STAD5_5_1 (EGG_SynC.cu:551-763).  Per lane: full cells add
+// the cell totals (EGG_SynC.cu:648-663); partial cells are streamed through
+// LDS in chunks of 64 points -- each lane loads one point and computes sin /
+// cos of v = x_q - a0 (a0 = the tile's cell corner) -- and every lane reads
+// each point as a broadcast and, for points within eps, adds
+// sin(x_q - x) = sin(v) cos(u) - cos(v) sin(u), u = x - a0.  Both angles are
+// below eps + 2 cell sides, so each term keeps an error of a few ulps of eps,
+// like a direct sinf(x_q - x).  Border lanes count their eps/2-neighbours in
+// other full cells from the same chunks (EGG_SynC.cu:666-681).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_update_tile(
-    const float *__restrict__ Xs, float *__restrict__ Xout, const int *__restrict__ perm_s,
-    int *__restrict__ perm_out, const int *__restrict__ cell_start,
+    const float *__restrict__ Xs, float *__restrict__ Xout, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
     const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ nonuni,
-    const int *__restrict__ tile_off, Geo g, DevState *st, DebugOut dbg) {
+    const int *__restrict__ tile_off, float *__restrict__ cres, int *__restrict__ cdbg, Geo g,
+    DevState *st, DebugOut dbg) {
     if (st->done) return;
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
     const int n = g.n, R = g.R, od = g.od;
@@ -138,17 +220,9 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     const int *cg_od = cell_g + (long long)od * n;
     bool unsynced = false;
     for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(&st->tile_next, 1);
-        t = __shfl(t, 0, kWave);
+        const int t = next_tile(&st->tile_next);
         if (t >= T) break;
-        int lo = 0, hi = C - 1;  // cell of tile t: largest c with tile_off[c] <= t
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (tile_off[mid] <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const int c0 = lo;
+        const int c0 = tile_cell(tile_off, C, t);
         const int s0c = cell_start[c0], s1c = cell_start[c0 + 1];
         const int own = s1c - s0c;
         const int i = s0c + (t - tile_off[c0]) * kTile + lane;
@@ -172,135 +246,160 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         }
         const int g0od = select_dim<D>(g0, od);
         int N = 0, half = 0;
-        Odo<D> odo;
-        odo.init(od, R);
-        while (!odo.finished) {
-            const int o = odo.outer(g0, g);
-            if (o >= 0) {
-                int cb = ob[o];
-                const int ce = oe[o];
-                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
-                for (int c = cb; c < ce; c++) {
-                    int gc[D];
+        walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
+            if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+            for (int c = cb; c < ce; c++) {
+                int gc[D];
+                bool stop;
+                const bool in = cell_in_window<D>(cell_g, n, c, g0, od, R + 1, g0od, gc, &stop);
+                if (stop) break;
+                if (!in) continue;
+                float sm, lg;
+                box_dist<D>(gc, x, g.cs, sm, lg);
+                const bool full = active && lg <= thr;
+                const bool part = active && !full && sm <= thr;
+                const int s0 = cell_start[c], s1 = cell_start[c + 1];
+                if (full) {
 #pragma unroll
-                    for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
-                    if (od < D && select_dim<D>(gc, od) > g0od + R + 1) break;
-                    bool skip = false;
-#pragma unroll
-                    for (int l = 0; l < D; l++)
-                        if (l > od && (gc[l] - g0[l] > R + 1 || g0[l] - gc[l] > R + 1)) skip = true;
-                    if (skip) continue;
-                    float sm, lg;
-                    box_dist<D>(gc, x, g.cs, sm, lg);
-                    const bool full = active && lg <= thr;
-                    const bool part = active && !full && sm <= thr;
-                    const int s0 = cell_start[c], s1 = cell_start[c + 1];
-                    if (full) {
+                    for (int l = 0; l < D; l++) {
+                        const float a = Ts[(long long)l * n + c] * cx[l];
+                        const float b = Tc[(long long)l * n + c] * sx[l];
+                        const float tt = a - b;
+                        sum[l] += tt;
+                    }
+                    N += s1 - s0;
+                    if (border && c == c0) half += s1 - s0;
+                }
+                const bool need_half = full && border && c != c0;
+                const bool any_part = __any(part);
+                if (!any_part && !__any(need_half)) continue;
+                if (!nonuni[c]) {
+                    // bitwise-equal points: one test decides for all, and the
+                    // same term is added once per point, in order
+                    float xq[D];
+                    const float s = dist2_to<D>(x, Xs, n, s0, xq);
+                    if (need_half && s <= thr_half) half += s1 - s0;
+                    if (part && s <= thr) {
 #pragma unroll
                         for (int l = 0; l < D; l++) {
-                            const float a = Ts[(long long)l * n + c] * cx[l];
-                            const float b = Tc[(long long)l * n + c] * sx[l];
-                            const float tt = a - b;
-                            sum[l] += tt;
+                            const float v = xq[l] - a0[l];
+                            const float p1 = sinf(v) * cu[l];
+                            const float p2 = cosf(v) * su[l];
+                            const float tt = p1 - p2;
+                            for (int j = s0; j < s1; j++) sml[l] += tt;
                         }
                         N += s1 - s0;
-                        if (border) {
-                            if (c == c0) {
-                                half += s1 - s0;
-                            } else if (!nonuni[c]) {
-                                // bitwise-equal points: one test decides for all
-                                float xq[D];
-                                if (dist2_to<D>(x, Xs, n, s0, xq) <= thr_half) half += s1 - s0;
-                            } else {
-                                for (int j = s0; j < s1; j++) {
-                                    float xq[D];
-                                    if (dist2_to<D>(x, Xs, n, j, xq) <= thr_half) half++;
-                                }
-                            }
-                        }
                     }
-                    if (!__any(part)) continue;
-                    if (!nonuni[c]) {
-                        // bitwise-equal points: the same test and the same term,
-                        // added once per point in order
-                        float xq[D];
-                        const float s = dist2_to<D>(x, Xs, n, s0, xq);
-                        if (part && s <= thr) {
+                    continue;
+                }
+                for (int base = s0; base < s1; base += kWave) {
+                    const int j = base + lane;
+                    if (j < s1) {
 #pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                const float v = xq[l] - a0[l];
-                                const float p1 = sinf(v) * cu[l];
-                                const float p2 = cosf(v) * su[l];
-                                const float tt = p1 - p2;
-                                for (int j = s0; j < s1; j++) sml[l] += tt;
-                            }
-                            N += s1 - s0;
-                        }
-                        continue;
-                    }
-                    for (int base = s0; base < s1; base += kWave) {
-                        const int j = base + lane;
-                        if (j < s1) {
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                const float xq = Xs[(long long)l * n + j];
+                        for (int l = 0; l < D; l++) {
+                            const float xq = Xs[(long long)l * n + j];
+                            src[lane].x[l] = xq;
+                            if (any_part) {
                                 const float v = xq - a0[l];
-                                src[lane].x[l] = xq;
                                 src[lane].sv[l] = sinf(v);
                                 src[lane].cv[l] = cosf(v);
                             }
                         }
-                        wave_sync();
-                        const int m = min(kWave, s1 - base);
-                        if (part) {
+                    }
+                    wave_sync();
+                    const int m = min(kWave, s1 - base);
+                    if (part) {
 #pragma unroll 4
-                            for (int k = 0; k < m; k++) {
-                                const SrcPt<D> &q = src[k];
-                                float s = 0.f;
+                        for (int k = 0; k < m; k++) {
+                            const SrcPt<D> &q = src[k];
+                            float s = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float df = x[l] - q.x[l];
+                                s += df * df;
+                            }
+                            if (s <= thr) {
 #pragma unroll
                                 for (int l = 0; l < D; l++) {
-                                    const float df = x[l] - q.x[l];
-                                    s += df * df;
+                                    const float p1 = q.sv[l] * cu[l];
+                                    const float p2 = q.cv[l] * su[l];
+                                    sml[l] += p1 - p2;
                                 }
-                                if (s <= thr) {
-#pragma unroll
-                                    for (int l = 0; l < D; l++) {
-                                        const float p1 = q.sv[l] * cu[l];
-                                        const float p2 = q.cv[l] * su[l];
-                                        sml[l] += p1 - p2;
-                                    }
-                                    N++;
-                                }
+                                N++;
                             }
                         }
-                        wave_sync();
+                    } else if (need_half) {
+                        for (int k = 0; k < m; k++) {
+                            float s = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float df = x[l] - src[k].x[l];
+                                s += df * df;
+                            }
+                            if (s <= thr_half) half++;
+                        }
                     }
+                    wave_sync();
                 }
             }
-            odo.next(od, R);
+        });
+        float xo[D];
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            float tt = sum[l] + sml[l];
+            tt = tt / (float)N;
+            xo[l] = x[l] + tt;
         }
+        const bool synced = !((!border && N != own) || (border && N != half));
         if (active) {
 #pragma unroll
-            for (int l = 0; l < D; l++) {
-                float tt = sum[l] + sml[l];
-                tt = tt / (float)N;
-                Xout[(long long)l * n + i] = x[l] + tt;
-            }
-            perm_out[i] = perm_s[i];
-            const bool synced = !((!border && N != own) || (border && N != half));
+            for (int l = 0; l < D; l++) Xout[(long long)l * n + i] = xo[l];
             if (!synced) unsynced = true;
             if (dbg.N) dbg.N[i] = N;
             if (dbg.half) dbg.half[i] = half;
             if (dbg.border) dbg.border[i] = border ? 1 : 0;
             if (dbg.synced) dbg.synced[i] = synced ? 1 : 0;
         }
+        // a single-tile cell of equal points larger than a tile: keep the
+        // result for the scatter pass
+        if (lane == 0 && own > kTile && !nonuni[c0]) {
+#pragma unroll
+            for (int l = 0; l < D; l++) cres[(long long)l * n + c0] = xo[l];
+            cdbg[4 * (long long)c0 + 0] = N;
+            cdbg[4 * (long long)c0 + 1] = half;
+            cdbg[4 * (long long)c0 + 2] = border ? 1 : 0;
+            cdbg[4 * (long long)c0 + 3] = synced ? 1 : 0;
+        }
     }
     if (__any(unsynced) && lane == 0) atomicAnd(&st->flag, 0);
 }
 
-// MBR_in_eps (EGG_SynC.cu:1037-1159) for the pair (p, q): per dimension the
-// coordinate, among q and q's eps/2-neighbours, closest to p; true if that
-// point is within eps of p.
+// Points beyond the first tile of a cell of equal points take the cell's
+// result; every point carries its id over to the next iteration's order.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_scatter_uniform(
+    const int *__restrict__ point_cell, const int *__restrict__ cell_start,
+    const int *__restrict__ nonuni, const float *__restrict__ cres, const int *__restrict__ cdbg,
+    const int *__restrict__ perm_s, float *__restrict__ Xout, int *__restrict__ perm_out, int n,
+    DevState *st, DebugOut dbg) {
+    if (st->done) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        perm_out[i] = perm_s[i];
+        const int c = point_cell[i];
+        if (i < cell_start[c] + kTile || nonuni[c]) continue;
+#pragma unroll
+        for (int l = 0; l < D; l++) Xout[(long long)l * n + i] = cres[(long long)l * n + c];
+        if (dbg.N) dbg.N[i] = cdbg[4 * (long long)c + 0];
+        if (dbg.half) dbg.half[i] = cdbg[4 * (long long)c + 1];
+        if (dbg.border) dbg.border[i] = cdbg[4 * (long long)c + 2];
+        if (dbg.synced) dbg.synced[i] = cdbg[4 * (long long)c + 3];
+    }
+}
+
+// MBR_in_eps (EGG_SynC.cu:1037-1159) for the pair (p, q), one thread: per
+// dimension the coordinate, among q and q's eps/2-neighbours, closest to p;
+// true if that point is within eps of p.  Used only when the pair queue of
+// the extra check overflows.
 template <int D>
 __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__restrict__ Xs,
                            const int *__restrict__ cell_start, const int *__restrict__ cell_g,
@@ -326,18 +425,13 @@ __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__rest
             if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
             for (int c = cb; c < ce; c++) {
                 int gc[D];
-#pragma unroll
-                for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
-                if (od < D && select_dim<D>(gc, od) > g0od + R + 1) break;
-                bool skip = false;
-#pragma unroll
-                for (int l = 0; l < D; l++)
-                    if (l > od && (gc[l] - g0[l] > R + 1 || g0[l] - gc[l] > R + 1)) skip = true;
-                if (skip) continue;
+                bool stop;
+                const bool in = cell_in_window<D>(cell_g, n, c, g0, od, R + 1, g0od, gc, &stop);
+                if (stop) break;
+                if (!in) continue;
                 float sm, lg;
                 box_dist<D>(gc, xq, g.cs, sm, lg);
                 if (!(sm <= g.thr_half)) continue;
-                // one representative of a cell of bitwise-equal points
                 const int j_end = nonuni[c] ? cell_start[c + 1] : cell_start[c] + 1;
                 for (int j = cell_start[c]; j < j_end; j++) {
                     float x2[D];
@@ -366,95 +460,177 @@ __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__rest
     return dd <= g.thr_eps;
 }
 
-// kernel_float_grid_extra_check_part1 (EGG_SynC.cu:900-1034): each point
-// looks for partners q with eps < |p-q| <= eps_extra in cells that are not
-// fully within eps and queues the pair for k_extra_mbr.  Pairs are
-// deduplicated by position (the MBR test depends on x_p and x_q only): a point
-// equal to its predecessor is skipped, and so is a partner equal to the last
-// one queued.  Should the queue overflow, the thread tests the pair itself.
-// Runs only when every point is locally synced; any hit clears the flag.
+// kernel_float_grid_extra_check_part1 (EGG_SynC.cu:900-1034), per tile: each
+// lane looks for partners q with eps < |p-q| <= eps_extra in cells that are
+// not fully within eps of it, reading the cells' points from LDS chunks, and
+// queues the pairs for k_extra_mbr.  Pairs are deduplicated by position (the
+// MBR test depends on x_p and x_q only): a point equal to its predecessor in
+// the cell is skipped, and so is a partner equal to the lane's last queued
+// one.  Should the queue overflow, the lane tests the pair itself.  Runs only
+// when every point is locally synced; any hit clears the flag.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_extra_check(
-    const float *__restrict__ Xs, const int *__restrict__ point_cell,
-    const int *__restrict__ cell_start, const int *__restrict__ cell_g,
-    const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ nonuni,
-    int *__restrict__ pairs, int pair_cap, Geo g, DevState *st) {
+__global__ __launch_bounds__(kBlock) void k_extra_tile(
+    const float *__restrict__ Xs, const int *__restrict__ cell_start,
+    const int *__restrict__ cell_g, const int *__restrict__ ob, const int *__restrict__ oe,
+    const int *__restrict__ nonuni, const int *__restrict__ tile_off, int *__restrict__ pairs,
+    int pair_cap, Geo g, DevState *st) {
     if (st->done || st->flag == 0) return;
+    __shared__ float s_x[kBlock / kWave][kWave][SrcPt<D>::P];
     const int n = g.n, od = g.od, R = g.R_extra;
+    const int C = st->C, T = st->T;
+    const int lane = __lane_id();
+    float(*src)[SrcPt<D>::P] = s_x[threadIdx.x / kWave];
     const int *cg_od = cell_g + (long long)od * n;
     bool hit = false;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n && !hit;
-         i += gridDim.x * blockDim.x) {
+    for (;;) {
+        const int t = next_tile(&st->xtile_next);
+        if (t >= T) break;
         if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+        const int c0 = tile_cell(tile_off, C, t);
+        const int s0c = cell_start[c0], s1c = cell_start[c0 + 1];
+        const int i = s0c + (t - tile_off[c0]) * kTile + lane;
+        bool active = i < s1c;
         float xp[D];
         int g0[D];
-        // the result depends on x_p only: a point bitwise equal to its
-        // predecessor in the same cell repeats the predecessor's result
-        const int ci = point_cell[i];
-        bool same = i > cell_start[ci];
+        bool same = active && i > s0c;
 #pragma unroll
         for (int l = 0; l < D; l++) {
-            xp[l] = Xs[(long long)l * n + i];
-            g0[l] = cell_coord(xp[l], g.cs, g.w);
-            if (same) same = __float_as_uint(xp[l]) == __float_as_uint(Xs[(long long)l * n + i - 1]);
+            g0[l] = cell_g[(long long)l * n + c0];
+            xp[l] = active ? Xs[(long long)l * n + i] : 0.f;
+            if (same)
+                same = __float_as_uint(xp[l]) == __float_as_uint(Xs[(long long)l * n + i - 1]);
         }
-        if (same) continue;
-        float lq[D];          // last partner whose MBR was tested (the MBR depends on x_q only)
+        active = active && !same;
+        if (!__any(active)) continue;
+        float lq[D];
         bool have_lq = false;
         const int g0od = select_dim<D>(g0, od);
-        Odo<D> odo;
-        odo.init(od, R);
-        while (!odo.finished && !hit) {
-            const int o = odo.outer(g0, g);
-            if (o >= 0) {
-                int cb = ob[o];
-                const int ce = oe[o];
-                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
-                for (int c = cb; c < ce && !hit; c++) {
-                    int gc[D];
+        auto consider = [&](const float *xq, int j) {
+            bool rep = have_lq;
 #pragma unroll
-                    for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
-                    if (od < D && select_dim<D>(gc, od) > g0od + R + 1) break;
-                    bool skip = false;
+            for (int l = 0; l < D; l++) rep = rep && __float_as_uint(lq[l]) == __float_as_uint(xq[l]);
+            if (rep) return;
 #pragma unroll
-                    for (int l = 0; l < D; l++)
-                        if (l > od && (gc[l] - g0[l] > R + 1 || g0[l] - gc[l] > R + 1)) skip = true;
-                    if (skip) continue;
-                    float sm, lg;
-                    box_dist<D>(gc, xp, g.cs, sm, lg);
-                    const bool included = sm <= g.thr_extra;
-                    const bool full = lg <= g.thr_eps;
-                    if (full || !included) continue;
-                    const int j_end = nonuni[c] ? cell_start[c + 1] : cell_start[c] + 1;
-                    for (int j = cell_start[c]; j < j_end; j++) {
-                        float xq[D];
-                        const float s = dist2_to<D>(xp, Xs, n, j, xq);
-                        if (!(s <= g.thr_eps) && s <= g.thr_extra) {
-                            bool rep = have_lq;
+            for (int l = 0; l < D; l++) lq[l] = xq[l];
+            have_lq = true;
+            const int slot = atomicAdd(&st->pair_count, 1);
+            if (slot < pair_cap) {
+                pairs[2 * slot] = i;
+                pairs[2 * slot + 1] = j;
+            } else if (mbr_in_eps<D>(xp, xq, Xs, cell_start, cell_g, ob, oe, nonuni, g)) {
+                hit = true;
+            }
+        };
+        walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
+            if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+            for (int c = cb; c < ce; c++) {
+                int gc[D];
+                bool stop;
+                const bool in = cell_in_window<D>(cell_g, n, c, g0, od, R + 1, g0od, gc, &stop);
+                if (stop) break;
+                if (!in) continue;
+                float sm, lg;
+                box_dist<D>(gc, xp, g.cs, sm, lg);
+                const bool need = active && !(lg <= g.thr_eps) && sm <= g.thr_extra;
+                if (!__any(need)) continue;
+                const int s0 = cell_start[c], s1 = cell_start[c + 1];
+                const int j_end = nonuni[c] ? s1 : s0 + 1;
+                for (int base = s0; base < j_end; base += kWave) {
+                    const int j = base + lane;
+                    if (j < j_end) {
 #pragma unroll
-                            for (int l = 0; l < D; l++)
-                                rep = rep && __float_as_uint(lq[l]) == __float_as_uint(xq[l]);
-                            if (rep) continue;
+                        for (int l = 0; l < D; l++) src[lane][l] = Xs[(long long)l * n + j];
+                    }
+                    wave_sync();
+                    const int m = min(kWave, j_end - base);
+                    if (need) {
+                        for (int k = 0; k < m; k++) {
+                            float xq[D], s = 0.f;
 #pragma unroll
-                            for (int l = 0; l < D; l++) lq[l] = xq[l];
-                            have_lq = true;
-                            const int slot = atomicAdd(&st->pair_count, 1);
-                            if (slot < pair_cap) {
-                                pairs[2 * slot] = i;
-                                pairs[2 * slot + 1] = j;
-                            } else if (mbr_in_eps<D>(xp, xq, Xs, cell_start, cell_g, ob, oe,
-                                                     nonuni, g)) {
-                                hit = true;
-                                break;
+                            for (int l = 0; l < D; l++) {
+                                xq[l] = src[k][l];
+                                const float df = xp[l] - xq[l];
+                                s += df * df;
                             }
+                            if (!(s <= g.thr_eps) && s <= g.thr_extra) consider(xq, base + k);
+                        }
+                    }
+                    wave_sync();
+                }
+            }
+        });
+    }
+    if (__any(hit) && lane == 0) atomicAnd(&st->flag, 0);
+}
+
+// kernel_float_grid_extra_check_part2 + MBR_in_eps (EGG_SynC.cu:1037-1203),
+// one wavefront per queued pair: the lanes share the scan of q's
+// eps/2-neighbourhood and keep, per dimension, the smallest squared distance
+// to p of a candidate coordinate (the reference keeps the coordinate with the
+// smallest one, so its MBR distance is the sum of these minima); a lane-min
+// reduction then gives the same value as the sequential scan.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_extra_mbr(
+    const float *__restrict__ Xs, const int *__restrict__ cell_start,
+    const int *__restrict__ cell_g, const int *__restrict__ ob, const int *__restrict__ oe,
+    const int *__restrict__ nonuni, const int *__restrict__ pairs, int pair_cap, Geo g,
+    DevState *st) {
+    if (st->done || st->flag == 0) return;
+    const int n = g.n, od = g.od, R = g.R_half;
+    const int npairs = min(st->pair_count, pair_cap);
+    const int lane = __lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    const int *cg_od = cell_g + (long long)od * n;
+    for (int k = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; k < npairs;
+         k += waves) {
+        if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+        const int ip = pairs[2 * k], iq = pairs[2 * k + 1];
+        float xp[D], xq[D], best[D];
+        int g0[D];
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            xp[l] = Xs[(long long)l * n + ip];
+            xq[l] = Xs[(long long)l * n + iq];
+            g0[l] = cell_coord(xq[l], g.cs, g.w);
+            float t = xq[l] - xp[l];  // the MBR starts at q
+            t *= t;
+            best[l] = t;
+        }
+        const int g0od = select_dim<D>(g0, od);
+        walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
+            if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+            for (int c = cb; c < ce; c++) {
+                int gc[D];
+                bool stop;
+                const bool in = cell_in_window<D>(cell_g, n, c, g0, od, R + 1, g0od, gc, &stop);
+                if (stop) break;
+                if (!in) continue;
+                float sm, lg;
+                box_dist<D>(gc, xq, g.cs, sm, lg);
+                if (!(sm <= g.thr_half)) continue;
+                const int j_end = nonuni[c] ? cell_start[c + 1] : cell_start[c] + 1;
+                for (int j = cell_start[c] + lane; j < j_end; j += kWave) {
+                    float x2[D];
+                    if (dist2_to<D>(xq, Xs, n, j, x2) <= g.thr_half) {
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            float nd = x2[l] - xp[l];
+                            nd *= nd;
+                            best[l] = fminf(best[l], nd);
                         }
                     }
                 }
             }
-            odo.next(od, R);
-        }
+        });
+#pragma unroll
+        for (int l = 0; l < D; l++)
+            for (int o = kWave / 2; o > 0; o >>= 1)
+                best[l] = fminf(best[l], __shfl_xor(best[l], o, kWave));
+        float dd = 0.f;
+#pragma unroll
+        for (int l = 0; l < D; l++) dd += best[l];
+        if (dd <= g.thr_eps && lane == 0) atomicAnd(&st->flag, 0);
     }
-    if (hit) atomicAnd(&st->flag, 0);
 }
 
 // SynC pass over the grid (SynC.cpp:16-56): every point within FP32 distance
@@ -561,92 +737,13 @@ __global__ __launch_bounds__(kBlock) void k_rc_finish(const double *__restrict__
     if (threadIdx.x == 0) st->rc = s_r[0] / (double)n;
 }
 
-// kernel_float_grid_extra_check_part2 + MBR_in_eps (EGG_SynC.cu:1037-1203),
-// one wavefront per queued pair: the lanes share the scan of q's
-// eps/2-neighbourhood and keep, per dimension, the smallest squared distance
-// to p of a candidate coordinate (the reference keeps the coordinate with the
-// smallest one, so its MBR distance is the sum of these minima); a lane-min
-// reduction then gives the same value as the sequential scan.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_extra_mbr(
-    const float *__restrict__ Xs, const int *__restrict__ cell_start,
-    const int *__restrict__ cell_g, const int *__restrict__ ob, const int *__restrict__ oe,
-    const int *__restrict__ nonuni, const int *__restrict__ pairs, int pair_cap, Geo g,
-    DevState *st) {
-    if (st->done || st->flag == 0) return;
-    const int n = g.n, od = g.od, R = g.R_half;
-    const int npairs = min(st->pair_count, pair_cap);
-    const int lane = __lane_id();
-    const int waves = gridDim.x * (blockDim.x / kWave);
-    const int *cg_od = cell_g + (long long)od * n;
-    for (int k = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; k < npairs;
-         k += waves) {
-        if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-        const int ip = pairs[2 * k], iq = pairs[2 * k + 1];
-        float xp[D], xq[D], best[D];
-        int g0[D];
-#pragma unroll
-        for (int l = 0; l < D; l++) {
-            xp[l] = Xs[(long long)l * n + ip];
-            xq[l] = Xs[(long long)l * n + iq];
-            g0[l] = cell_coord(xq[l], g.cs, g.w);
-            float t = xq[l] - xp[l];  // the MBR starts at q
-            t *= t;
-            best[l] = t;
-        }
-        const int g0od = select_dim<D>(g0, od);
-        Odo<D> odo;
-        odo.init(od, R);
-        while (!odo.finished) {
-            const int o = odo.outer(g0, g);
-            if (o >= 0) {
-                int cb = ob[o];
-                const int ce = oe[o];
-                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
-                for (int c = cb; c < ce; c++) {
-                    int gc[D];
-#pragma unroll
-                    for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
-                    if (od < D && select_dim<D>(gc, od) > g0od + R + 1) break;
-                    bool skip = false;
-#pragma unroll
-                    for (int l = 0; l < D; l++)
-                        if (l > od && (gc[l] - g0[l] > R + 1 || g0[l] - gc[l] > R + 1)) skip = true;
-                    if (skip) continue;
-                    float sm, lg;
-                    box_dist<D>(gc, xq, g.cs, sm, lg);
-                    if (!(sm <= g.thr_half)) continue;
-                    const int j_end = nonuni[c] ? cell_start[c + 1] : cell_start[c] + 1;
-                    for (int j = cell_start[c] + lane; j < j_end; j += kWave) {
-                        float x2[D];
-                        if (dist2_to<D>(xq, Xs, n, j, x2) <= g.thr_half) {
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                float nd = x2[l] - xp[l];
-                                nd *= nd;
-                                best[l] = fminf(best[l], nd);
-                            }
-                        }
-                    }
-                }
-            }
-            odo.next(od, R);
-        }
-#pragma unroll
-        for (int l = 0; l < D; l++)
-            for (int o = kWave / 2; o > 0; o >>= 1) best[l] = fminf(best[l], __shfl_xor(best[l], o, kWave));
-        float dd = 0.f;
-#pragma unroll
-        for (int l = 0; l < D; l++) dd += best[l];
-        if (dd <= g.thr_eps && lane == 0) atomicAnd(&st->flag, 0);
-    }
-}
 
 __global__ void k_set_flag(DevState *st) {
     if (st->done) return;
     st->flag = 1;
     st->pair_count = 0;
     st->tile_next = 0;
+    st->xtile_next = 0;
 }
 
 // End of an iteration (EGG_SynC.cu:1322,1425 / SynC.cpp:277).
@@ -672,18 +769,25 @@ static inline int grid_for(long long n) {
 }
 
 // Enough workgroups to fill the chip; they drain the tile queue and exit.
-constexpr int kUpdateBlocks = 2048;
+constexpr int kQueueBlocks = 2048;
+
+static inline int queue_blocks(const Geo &g) {
+    const long long tiles_max = (long long)g.n / kTile + g.n;  // >= n/64 + C
+    return (int)std::min<long long>(kQueueBlocks, (tiles_max + 3) / 4);
+}
 
 void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *dbg,
                        hipStream_t s) {
     DebugOut d = dbg ? *dbg : DebugOut{nullptr, nullptr, nullptr, nullptr};
-    const long long tiles_max = (long long)g.n / kTile + g.n;  // <= n/64 + C
-    const int gb = (int)std::min<long long>(kUpdateBlocks, (tiles_max + 3) / 4);
+    const int qb = queue_blocks(g);
 #define UPD_CASE(DD)                                                                          \
     case DD:                                                                                  \
-        k_update_tile<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.X, b.perm_s, b.perm, b.cell_start,    \
-                                                b.cell_g, b.Ts, b.Tc, b.ob, b.oe, b.nonuni,   \
-                                                b.tile_off, g, st, d);                        \
+        k_update_tile<DD><<<qb, kBlock, 0, s>>>(b.Xs, b.X, b.cell_start, b.cell_g, b.Ts, b.Tc, \
+                                                b.ob, b.oe, b.nonuni, b.tile_off, b.cres,     \
+                                                b.cdbg, g, st, d);                            \
+        k_scatter_uniform<DD><<<grid_for(g.n), kBlock, 0, s>>>(                               \
+            b.point_cell, b.cell_start, b.nonuni, b.cres, b.cdbg, b.perm_s, b.X, b.perm, g.n, \
+            st, d);                                                                           \
         break;
     switch (g.d) {
         UPD_CASE(1) UPD_CASE(2) UPD_CASE(3) UPD_CASE(4)
@@ -693,14 +797,14 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
 }
 
 void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s) {
-    const int gb = grid_for(g.n);
-#define EX_CASE(DD)                                                                       \
-    case DD:                                                                              \
-        k_extra_check<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.point_cell, b.cell_start, b.cell_g, \
-                                                b.ob, b.oe, b.nonuni, b.pairs, b.pair_cap,  \
-                                                g, st);                                     \
-        k_extra_mbr<DD><<<1024, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.ob, b.oe,   \
-                                                b.nonuni, b.pairs, b.pair_cap, g, st);      \
+    const int qb = queue_blocks(g);
+#define EX_CASE(DD)                                                                           \
+    case DD:                                                                                  \
+        k_extra_tile<DD><<<qb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.ob, b.oe,      \
+                                               b.nonuni, b.tile_off, b.pairs, b.pair_cap, g,  \
+                                               st);                                           \
+        k_extra_mbr<DD><<<1024, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.ob, b.oe,     \
+                                                b.nonuni, b.pairs, b.pair_cap, g, st);        \
         break;
     switch (g.d) {
         EX_CASE(1) EX_CASE(2) EX_CASE(3) EX_CASE(4)
