@@ -68,7 +68,7 @@ struct DevState {
     int T;             // update tiles (<= 64 points of one cell each)
     int tile_next;     // work-queue head of the update kernel
     int xtile_next;    // work-queue head of the extra check
-    int pad;
+    int runs;          // runs of equal positions in the sorted order
     double rc;         // order parameter of the last r_c evaluation
 };
 

@@ -211,30 +211,65 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
 __global__ __launch_bounds__(kBlock) void k_wi_count(const int *__restrict__ cell_start,
                                                      const DevState *st,
                                                      int *__restrict__ nchunk,
-                                                     int *__restrict__ nonuni,
                                                      DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int C = st->C;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
         const int cnt = cell_start[c + 1] - cell_start[c];
         nchunk[c] = (cnt + kSumChunk - 1) / kSumChunk;
-        nonuni[c] = 0;
     }
 }
 
-// Update tiles per cell: ceil(cnt / 64), or one for a cell of equal points
-// (all its points get the same result).
-__global__ __launch_bounds__(kBlock) void k_tile_count(const int *__restrict__ cell_start,
-                                                       const int *__restrict__ nonuni,
+// Run heads: a sorted position starts a run when its position differs
+// bitwise from its predecessor's (equal positions always share a cell).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_run_heads(const float *__restrict__ Xs, int n,
+                                                      int *__restrict__ head,
+                                                      DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        bool h = i == 0;
+#pragma unroll
+        for (int l = 0; l < D; l++)
+            if (i > 0)
+                h |= __float_as_uint(Xs[(long long)l * n + i]) !=
+                     __float_as_uint(Xs[(long long)l * n + i - 1]);
+        head[i] = h ? 1 : 0;
+    }
+}
+
+// Run table from the run-head scan: run_start[r], cell_run[c] (first run of
+// each cell), run_start[runs] = n, cell_run[C] = runs.
+__global__ __launch_bounds__(kBlock) void k_runs(const int *__restrict__ head,
+                                                 const int *__restrict__ hpos,
+                                                 const int *__restrict__ point_cell,
+                                                 const int *__restrict__ cell_start, int n,
+                                                 int *__restrict__ run_start,
+                                                 int *__restrict__ cell_run, DevState *st,
+                                                 DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int r = hpos[i] + head[i] - 1;
+        if (head[i]) run_start[r] = i;
+        const int c = point_cell[i];
+        if (cell_start[c] == i) cell_run[c] = r;
+        if (i == n - 1) {
+            run_start[r + 1] = n;
+            cell_run[c + 1] = r + 1;
+            st->runs = r + 1;
+        }
+    }
+}
+
+// Update tiles per cell: one per 64 runs.
+__global__ __launch_bounds__(kBlock) void k_tile_count(const int *__restrict__ cell_run,
                                                        const DevState *st,
                                                        int *__restrict__ ntile,
                                                        DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int C = st->C;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
-        const int cnt = cell_start[c + 1] - cell_start[c];
-        ntile[c] = nonuni[c] ? (cnt + kTile - 1) / kTile : 1;
-    }
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x)
+        ntile[c] = (cell_run[c + 1] - cell_run[c] + kTile - 1) / kTile;
 }
 
 // One wavefront per work item (<= kSumChunk points of one cell).  Per dim:
@@ -247,8 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_summary(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const int *__restrict__ wi_off,
     const int *__restrict__ nchunk, Geo g, const DevState *st, float *__restrict__ Ts,
-    float *__restrict__ Tc, float *__restrict__ partial, int *__restrict__ nonuni,
-    DevState *st_loop) {
+    float *__restrict__ Tc, float *__restrict__ partial, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int n = g.n, C = st->C, W = st->W;
     const int lane = __lane_id();
@@ -266,15 +300,6 @@ __global__ __launch_bounds__(kBlock) void k_summary(
         const int j0 = c0 + k * kSumChunk;
         const int j1 = min(c1, j0 + kSumChunk);
         const int nch = nchunk[c];
-        // uniform-cell flag: does any point differ bitwise from the cell's first point?
-        bool differs = false;
-        for (int j = j0 + lane; j < j1; j += kWave) {
-#pragma unroll
-            for (int l = 0; l < D; l++)
-                differs |= __float_as_uint(Xs[(long long)l * n + j]) !=
-                           __float_as_uint(Xs[(long long)l * n + c0]);
-        }
-        if (__any(differs) && lane == 0) atomicOr(&nonuni[c], 1);
 #pragma unroll
         for (int l = 0; l < D; l++) {
             const float a = (float)cell_g[(long long)l * n + c] * g.cs;
@@ -366,7 +391,7 @@ __global__ void k_store_total(const int *__restrict__ off, const int *__restrict
 void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
     const int n = g.n;
     const int gb = grid_for(n);
-    k_wi_count<<<gb, kBlock, 0, s>>>(b.cell_start, st, b.nchunk, b.nonuni, st_loop);
+    k_wi_count<<<gb, kBlock, 0, s>>>(b.cell_start, st, b.nchunk, st_loop);
     scan_exclusive(b.nchunk, b.wi_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
     k_store_total<<<1, 1, 0, s>>>(b.wi_off, b.nchunk, st, &st->W, st_loop);
     // work items <= C + n / kSumChunk <= n + n / kSumChunk
@@ -375,8 +400,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
 #define SUM_CASE(DD)                                                                         \
     case DD:                                                                                 \
         k_summary<DD><<<wb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.wi_off,          \
-                                            b.nchunk, g, st, b.Ts, b.Tc, b.partial, b.nonuni, \
-                                            st_loop);                                         \
+                                            b.nchunk, g, st, b.Ts, b.Tc, b.partial, st_loop); \
         k_summary_combine<DD><<<gb, kBlock, 0, s>>>(b.cell_start, b.cell_g, b.wi_off,        \
                                                     b.nchunk, g, st, b.Ts, b.Tc, b.partial,  \
                                                     st_loop);                                \
@@ -386,7 +410,18 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
         SUM_CASE(5) SUM_CASE(6) SUM_CASE(7) SUM_CASE(8)
     }
 #undef SUM_CASE
-    k_tile_count<<<gb, kBlock, 0, s>>>(b.cell_start, b.nonuni, st, b.ntile, st_loop);
+    // runs of equal positions and the update tiles over them
+#define RUN_CASE(DD) \
+    case DD: k_run_heads<DD><<<gb, kBlock, 0, s>>>(b.Xs, n, b.head, st_loop); break;
+    switch (g.d) {
+        RUN_CASE(1) RUN_CASE(2) RUN_CASE(3) RUN_CASE(4)
+        RUN_CASE(5) RUN_CASE(6) RUN_CASE(7) RUN_CASE(8)
+    }
+#undef RUN_CASE
+    scan_exclusive(b.head, b.hpos, n, nullptr, b.scan_tmp, nullptr, st_loop, s);
+    k_runs<<<gb, kBlock, 0, s>>>(b.head, b.hpos, b.point_cell, b.cell_start, n, b.run_start,
+                                 b.cell_run, st, st_loop);
+    k_tile_count<<<gb, kBlock, 0, s>>>(b.cell_run, st, b.ntile, st_loop);
     scan_exclusive(b.ntile, b.tile_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
     k_store_total<<<1, 1, 0, s>>>(b.tile_off, b.ntile, st, &st->T, st_loop);
 }

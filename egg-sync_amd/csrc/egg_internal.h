@@ -37,10 +37,13 @@ struct GridBufs {
     int *cell_outer;     // [n]
     int *ob, *oe;        // [O]
     int *nchunk, *wi_off;// [n]
-    int *nonuni;         // [n]   1 if a cell holds two bitwise-different points
-    int *ntile, *tile_off;// [n]  update tiles per cell, exclusive scan
-    float *cres;         // [d][n] result of a single-tile cell of equal points
-    int *cdbg;           // [4n]  its N / half / border / synced
+    // runs: maximal stretches of bitwise-equal consecutive positions (sorted
+    // order); head / hpos are reused for the run-head flags and their scan
+    int *run_start;      // [n+1] first sorted position of each run
+    int *cell_run;       // [n+1] first run of each cell
+    int *ntile, *tile_off;// [n]  update tiles per cell (<= 64 runs each), exclusive scan
+    float *rres;         // [d][n] new position of each run
+    int *rdbg;           // [4n]  N / half / border / synced of each run (diagnostics)
     float *Ts, *Tc;      // [d][n]
     float *partial;      // [n + n/kSumChunk + 1][4*d]
     int *pairs;          // [2 * pair_cap] extra-check (p, q) sorted positions

@@ -160,11 +160,12 @@ Layout carve(void *base, const Geo &g) {
     b.oe = c.take<int>((size_t)g.O);
     b.nchunk = c.take<int>(n);
     b.wi_off = c.take<int>(n);
-    b.nonuni = c.take<int>(n);
+    b.run_start = c.take<int>(n + 1);
+    b.cell_run = c.take<int>(n + 1);
     b.ntile = c.take<int>(n);
     b.tile_off = c.take<int>(n);
-    b.cres = c.take<float>(n * d);
-    b.cdbg = c.take<int>(4 * n);
+    b.rres = c.take<float>(n * d);
+    b.rdbg = c.take<int>(4 * n);
     b.Ts = c.take<float>(n * d);
     b.Tc = c.take<float>(n * d);
     b.partial = c.take<float>((n + n / kSumChunk + 1) * 4 * d);

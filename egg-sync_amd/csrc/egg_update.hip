@@ -1,16 +1,19 @@
 // Neighbour update, local-sync flag, extra check and FP64 order parameter
 // (EGG_SynC.cu:551-763, 767-1203; SynC.cpp:30-56), gfx950.
 //
-// The EGG kernels work on tiles: one wavefront per tile of <= 64 points of one
-// cell, tiles taken from a work queue.  All lanes of a tile share the cell, so
-// the walk over candidate cells and every cell load are wave-uniform while
-// each lane runs the reference's per-point tests.  Candidate cells come from
-// the outer-cell box of radius R (as the reference), pruned in the remaining
-// dimensions to |dg| <= R+1 by a binary search over the column's cells (sorted
-// by g_od): cells farther than that fail the reference's box test by more than
-// a cell side, so results do not change.  Cells whose points are all bitwise
-// equal form a single tile: every point of such a cell has the same inputs and
-// so the same result, which a scatter pass copies to the rest of the cell.
+// The EGG kernels work on runs -- maximal stretches of bitwise-equal
+// consecutive positions in the cell-sorted order: every point of a run has
+// the same inputs and hence the same result, and every scan over a cell's
+// points can visit its runs once, counting each with its length.  One
+// wavefront handles a tile of <= 64 runs of one cell (tiles come from a work
+// queue), so the walk over candidate cells and every cell load are
+// wave-uniform while each lane runs the reference's per-point tests on its
+// run's position.  A scatter pass copies each run's result to its points.
+// Candidate cells come from the outer-cell box of radius R (as the
+// reference), pruned in the remaining dimensions to |dg| <= R+1 by a binary
+// search over the column's cells (sorted by g_od): cells farther than that
+// fail the reference's box test by more than a cell side, so results do not
+// change.
 #include <algorithm>
 
 #include "egg_internal.h"
@@ -161,8 +164,8 @@ __device__ __forceinline__ bool cell_in_window(const int *__restrict__ cell_g, i
     return in && !*stop;
 }
 
-// A source point of a cell staged in LDS: its coordinates and the sine /
-// cosine of its offset from the tile's cell corner.
+// A source run of a cell staged in LDS: its position, the sine / cosine of
+// its offset from the tile's cell corner, and its length.
 template <int D>
 struct SrcPt {
     static constexpr int P = (D + 3) / 4 * 4;
@@ -205,28 +208,30 @@ __device__ __forceinline__ int next_tile(int *counter) {
 // other full cells from the same chunks (EGG_SynC.cu:666-681).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_update_tile(
-    const float *__restrict__ Xs, float *__restrict__ Xout, const int *__restrict__ cell_start,
+    const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
-    const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ nonuni,
-    const int *__restrict__ tile_off, float *__restrict__ cres, int *__restrict__ cdbg, Geo g,
-    DevState *st, DebugOut dbg) {
+    const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ run_start,
+    const int *__restrict__ cell_run, const int *__restrict__ tile_off,
+    float *__restrict__ rres, int *__restrict__ rdbg, Geo g, DevState *st) {
     if (st->done) return;
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
+    __shared__ int s_len[kBlock / kWave][kWave];
     const int n = g.n, R = g.R, od = g.od;
     const int C = st->C, T = st->T;
     const float thr = g.thr_eps, thr_half = g.thr_half;
     const int lane = __lane_id();
     SrcPt<D> *src = s_src[threadIdx.x / kWave];
+    int *len = s_len[threadIdx.x / kWave];
     const int *cg_od = cell_g + (long long)od * n;
     bool unsynced = false;
     for (;;) {
         const int t = next_tile(&st->tile_next);
         if (t >= T) break;
         const int c0 = tile_cell(tile_off, C, t);
-        const int s0c = cell_start[c0], s1c = cell_start[c0 + 1];
-        const int own = s1c - s0c;
-        const int i = s0c + (t - tile_off[c0]) * kTile + lane;
-        const bool active = i < s1c;
+        const int own = cell_start[c0 + 1] - cell_start[c0];
+        const int r = cell_run[c0] + (t - tile_off[c0]) * kTile + lane;
+        const bool active = r < cell_run[c0 + 1];
+        const int i = active ? run_start[r] : 0;
         float x[D], a0[D], sx[D], cx[D], su[D], cu[D], sum[D], sml[D];
         int g0[D];
         bool border = false;
@@ -258,7 +263,6 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                 box_dist<D>(gc, x, g.cs, sm, lg);
                 const bool full = active && lg <= thr;
                 const bool part = active && !full && sm <= thr;
-                const int s0 = cell_start[c], s1 = cell_start[c + 1];
                 if (full) {
 #pragma unroll
                     for (int l = 0; l < D; l++) {
@@ -267,34 +271,22 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                         const float tt = a - b;
                         sum[l] += tt;
                     }
-                    N += s1 - s0;
-                    if (border && c == c0) half += s1 - s0;
+                    const int cnt = cell_start[c + 1] - cell_start[c];
+                    N += cnt;
+                    if (border && c == c0) half += cnt;
                 }
                 const bool need_half = full && border && c != c0;
                 const bool any_part = __any(part);
                 if (!any_part && !__any(need_half)) continue;
-                if (!nonuni[c]) {
-                    // bitwise-equal points: one test decides for all, and the
-                    // same term is added once per point, in order
-                    float xq[D];
-                    const float s = dist2_to<D>(x, Xs, n, s0, xq);
-                    if (need_half && s <= thr_half) half += s1 - s0;
-                    if (part && s <= thr) {
-#pragma unroll
-                        for (int l = 0; l < D; l++) {
-                            const float v = xq[l] - a0[l];
-                            const float p1 = sinf(v) * cu[l];
-                            const float p2 = cosf(v) * su[l];
-                            const float tt = p1 - p2;
-                            for (int j = s0; j < s1; j++) sml[l] += tt;
-                        }
-                        N += s1 - s0;
-                    }
-                    continue;
-                }
-                for (int base = s0; base < s1; base += kWave) {
-                    const int j = base + lane;
-                    if (j < s1) {
+                // the cell's runs, 64 per chunk: one test per run, each
+                // counted with its length; a run's term is added once per
+                // point, in order, as a pointwise loop would
+                const int r0 = cell_run[c], r1 = cell_run[c + 1];
+                for (int base = r0; base < r1; base += kWave) {
+                    const int rr = base + lane;
+                    if (rr < r1) {
+                        const int j = run_start[rr];
+                        len[lane] = run_start[rr + 1] - j;
 #pragma unroll
                         for (int l = 0; l < D; l++) {
                             const float xq = Xs[(long long)l * n + j];
@@ -307,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                         }
                     }
                     wave_sync();
-                    const int m = min(kWave, s1 - base);
+                    const int m = min(kWave, r1 - base);
                     if (part) {
 #pragma unroll 4
                         for (int k = 0; k < m; k++) {
@@ -319,13 +311,15 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                                 s += df * df;
                             }
                             if (s <= thr) {
+                                const int lk = len[k];
 #pragma unroll
                                 for (int l = 0; l < D; l++) {
                                     const float p1 = q.sv[l] * cu[l];
                                     const float p2 = q.cv[l] * su[l];
-                                    sml[l] += p1 - p2;
+                                    const float tt = p1 - p2;
+                                    for (int e = 0; e < lk; e++) sml[l] += tt;
                                 }
-                                N++;
+                                N += lk;
                             }
                         }
                     } else if (need_half) {
@@ -336,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                                 const float df = x[l] - src[k].x[l];
                                 s += df * df;
                             }
-                            if (s <= thr_half) half++;
+                            if (s <= thr_half) half += len[k];
                         }
                     }
                     wave_sync();
@@ -353,46 +347,38 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         const bool synced = !((!border && N != own) || (border && N != half));
         if (active) {
 #pragma unroll
-            for (int l = 0; l < D; l++) Xout[(long long)l * n + i] = xo[l];
+            for (int l = 0; l < D; l++) rres[(long long)l * n + r] = xo[l];
             if (!synced) unsynced = true;
-            if (dbg.N) dbg.N[i] = N;
-            if (dbg.half) dbg.half[i] = half;
-            if (dbg.border) dbg.border[i] = border ? 1 : 0;
-            if (dbg.synced) dbg.synced[i] = synced ? 1 : 0;
-        }
-        // a single-tile cell of equal points larger than a tile: keep the
-        // result for the scatter pass
-        if (lane == 0 && own > kTile && !nonuni[c0]) {
-#pragma unroll
-            for (int l = 0; l < D; l++) cres[(long long)l * n + c0] = xo[l];
-            cdbg[4 * (long long)c0 + 0] = N;
-            cdbg[4 * (long long)c0 + 1] = half;
-            cdbg[4 * (long long)c0 + 2] = border ? 1 : 0;
-            cdbg[4 * (long long)c0 + 3] = synced ? 1 : 0;
+            if (rdbg) {
+                rdbg[4 * (long long)r + 0] = N;
+                rdbg[4 * (long long)r + 1] = half;
+                rdbg[4 * (long long)r + 2] = border ? 1 : 0;
+                rdbg[4 * (long long)r + 3] = synced ? 1 : 0;
+            }
         }
     }
     if (__any(unsynced) && lane == 0) atomicAnd(&st->flag, 0);
 }
 
-// Points beyond the first tile of a cell of equal points take the cell's
-// result; every point carries its id over to the next iteration's order.
+// Every point takes its run's result and carries its id over to the next
+// iteration's order.  The run of sorted position i is hpos[i] + head[i] - 1.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_scatter_uniform(
-    const int *__restrict__ point_cell, const int *__restrict__ cell_start,
-    const int *__restrict__ nonuni, const float *__restrict__ cres, const int *__restrict__ cdbg,
-    const int *__restrict__ perm_s, float *__restrict__ Xout, int *__restrict__ perm_out, int n,
-    DevState *st, DebugOut dbg) {
+__global__ __launch_bounds__(kBlock) void k_scatter_runs(
+    const int *__restrict__ head, const int *__restrict__ hpos, const float *__restrict__ rres,
+    const int *__restrict__ rdbg, const int *__restrict__ perm_s, float *__restrict__ Xout,
+    int *__restrict__ perm_out, int n, DevState *st, DebugOut dbg) {
     if (st->done) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         perm_out[i] = perm_s[i];
-        const int c = point_cell[i];
-        if (i < cell_start[c] + kTile || nonuni[c]) continue;
+        const int r = hpos[i] + head[i] - 1;
 #pragma unroll
-        for (int l = 0; l < D; l++) Xout[(long long)l * n + i] = cres[(long long)l * n + c];
-        if (dbg.N) dbg.N[i] = cdbg[4 * (long long)c + 0];
-        if (dbg.half) dbg.half[i] = cdbg[4 * (long long)c + 1];
-        if (dbg.border) dbg.border[i] = cdbg[4 * (long long)c + 2];
-        if (dbg.synced) dbg.synced[i] = cdbg[4 * (long long)c + 3];
+        for (int l = 0; l < D; l++) Xout[(long long)l * n + i] = rres[(long long)l * n + r];
+        if (rdbg) {
+            if (dbg.N) dbg.N[i] = rdbg[4 * (long long)r + 0];
+            if (dbg.half) dbg.half[i] = rdbg[4 * (long long)r + 1];
+            if (dbg.border) dbg.border[i] = rdbg[4 * (long long)r + 2];
+            if (dbg.synced) dbg.synced[i] = rdbg[4 * (long long)r + 3];
+        }
     }
 }
 
@@ -402,9 +388,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter_uniform(
 // the extra check overflows.
 template <int D>
 __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__restrict__ Xs,
-                           const int *__restrict__ cell_start, const int *__restrict__ cell_g,
-                           const int *__restrict__ ob, const int *__restrict__ oe,
-                           const int *__restrict__ nonuni, const Geo &g) {
+                           const int *__restrict__ cell_g, const int *__restrict__ ob,
+                           const int *__restrict__ oe, const int *__restrict__ run_start,
+                           const int *__restrict__ cell_run, const Geo &g) {
     const int n = g.n, od = g.od, R = g.R_half;
     float mbr[D];
     int g0[D];
@@ -432,10 +418,9 @@ __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__rest
                 float sm, lg;
                 box_dist<D>(gc, xq, g.cs, sm, lg);
                 if (!(sm <= g.thr_half)) continue;
-                const int j_end = nonuni[c] ? cell_start[c + 1] : cell_start[c] + 1;
-                for (int j = cell_start[c]; j < j_end; j++) {
+                for (int rr = cell_run[c]; rr < cell_run[c + 1]; rr++) {
                     float x2[D];
-                    if (dist2_to<D>(xq, Xs, n, j, x2) <= g.thr_half) {
+                    if (dist2_to<D>(xq, Xs, n, run_start[rr], x2) <= g.thr_half) {
 #pragma unroll
                         for (int l = 0; l < D; l++) {
                             float od_ = mbr[l] - xp[l];
@@ -470,16 +455,18 @@ __device__ bool mbr_in_eps(const float *xp, const float *xq, const float *__rest
 // when every point is locally synced; any hit clears the flag.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_extra_tile(
-    const float *__restrict__ Xs, const int *__restrict__ cell_start,
-    const int *__restrict__ cell_g, const int *__restrict__ ob, const int *__restrict__ oe,
-    const int *__restrict__ nonuni, const int *__restrict__ tile_off, int *__restrict__ pairs,
+    const float *__restrict__ Xs, const int *__restrict__ cell_g, const int *__restrict__ ob,
+    const int *__restrict__ oe, const int *__restrict__ run_start,
+    const int *__restrict__ cell_run, const int *__restrict__ tile_off, int *__restrict__ pairs,
     int pair_cap, Geo g, DevState *st) {
     if (st->done || st->flag == 0) return;
     __shared__ float s_x[kBlock / kWave][kWave][SrcPt<D>::P];
+    __shared__ int s_pos[kBlock / kWave][kWave];
     const int n = g.n, od = g.od, R = g.R_extra;
     const int C = st->C, T = st->T;
     const int lane = __lane_id();
     float(*src)[SrcPt<D>::P] = s_x[threadIdx.x / kWave];
+    int *src_pos = s_pos[threadIdx.x / kWave];
     const int *cg_od = cell_g + (long long)od * n;
     bool hit = false;
     for (;;) {
@@ -487,21 +474,16 @@ __global__ __launch_bounds__(kBlock) void k_extra_tile(
         if (t >= T) break;
         if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
         const int c0 = tile_cell(tile_off, C, t);
-        const int s0c = cell_start[c0], s1c = cell_start[c0 + 1];
-        const int i = s0c + (t - tile_off[c0]) * kTile + lane;
-        bool active = i < s1c;
+        const int r = cell_run[c0] + (t - tile_off[c0]) * kTile + lane;
+        const bool active = r < cell_run[c0 + 1];
+        const int i = active ? run_start[r] : 0;
         float xp[D];
         int g0[D];
-        bool same = active && i > s0c;
 #pragma unroll
         for (int l = 0; l < D; l++) {
             g0[l] = cell_g[(long long)l * n + c0];
             xp[l] = active ? Xs[(long long)l * n + i] : 0.f;
-            if (same)
-                same = __float_as_uint(xp[l]) == __float_as_uint(Xs[(long long)l * n + i - 1]);
         }
-        active = active && !same;
-        if (!__any(active)) continue;
         float lq[D];
         bool have_lq = false;
         const int g0od = select_dim<D>(g0, od);
@@ -517,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_extra_tile(
             if (slot < pair_cap) {
                 pairs[2 * slot] = i;
                 pairs[2 * slot + 1] = j;
-            } else if (mbr_in_eps<D>(xp, xq, Xs, cell_start, cell_g, ob, oe, nonuni, g)) {
+            } else if (mbr_in_eps<D>(xp, xq, Xs, cell_g, ob, oe, run_start, cell_run, g)) {
                 hit = true;
             }
         };
@@ -533,16 +515,17 @@ __global__ __launch_bounds__(kBlock) void k_extra_tile(
                 box_dist<D>(gc, xp, g.cs, sm, lg);
                 const bool need = active && !(lg <= g.thr_eps) && sm <= g.thr_extra;
                 if (!__any(need)) continue;
-                const int s0 = cell_start[c], s1 = cell_start[c + 1];
-                const int j_end = nonuni[c] ? s1 : s0 + 1;
-                for (int base = s0; base < j_end; base += kWave) {
-                    const int j = base + lane;
-                    if (j < j_end) {
+                const int r0 = cell_run[c], r1 = cell_run[c + 1];
+                for (int base = r0; base < r1; base += kWave) {
+                    const int rr = base + lane;
+                    if (rr < r1) {
+                        const int j = run_start[rr];
+                        src_pos[lane] = j;
 #pragma unroll
                         for (int l = 0; l < D; l++) src[lane][l] = Xs[(long long)l * n + j];
                     }
                     wave_sync();
-                    const int m = min(kWave, j_end - base);
+                    const int m = min(kWave, r1 - base);
                     if (need) {
                         for (int k = 0; k < m; k++) {
                             float xq[D], s = 0.f;
@@ -552,7 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_extra_tile(
                                 const float df = xp[l] - xq[l];
                                 s += df * df;
                             }
-                            if (!(s <= g.thr_eps) && s <= g.thr_extra) consider(xq, base + k);
+                            if (!(s <= g.thr_eps) && s <= g.thr_extra) consider(xq, src_pos[k]);
                         }
                     }
                     wave_sync();
@@ -571,9 +554,9 @@ __global__ __launch_bounds__(kBlock) void k_extra_tile(
 // reduction then gives the same value as the sequential scan.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_extra_mbr(
-    const float *__restrict__ Xs, const int *__restrict__ cell_start,
-    const int *__restrict__ cell_g, const int *__restrict__ ob, const int *__restrict__ oe,
-    const int *__restrict__ nonuni, const int *__restrict__ pairs, int pair_cap, Geo g,
+    const float *__restrict__ Xs, const int *__restrict__ cell_g, const int *__restrict__ ob,
+    const int *__restrict__ oe, const int *__restrict__ run_start,
+    const int *__restrict__ cell_run, const int *__restrict__ pairs, int pair_cap, Geo g,
     DevState *st) {
     if (st->done || st->flag == 0) return;
     const int n = g.n, od = g.od, R = g.R_half;
@@ -608,10 +591,9 @@ __global__ __launch_bounds__(kBlock) void k_extra_mbr(
                 float sm, lg;
                 box_dist<D>(gc, xq, g.cs, sm, lg);
                 if (!(sm <= g.thr_half)) continue;
-                const int j_end = nonuni[c] ? cell_start[c + 1] : cell_start[c] + 1;
-                for (int j = cell_start[c] + lane; j < j_end; j += kWave) {
+                for (int rr = cell_run[c] + lane; rr < cell_run[c + 1]; rr += kWave) {
                     float x2[D];
-                    if (dist2_to<D>(xq, Xs, n, j, x2) <= g.thr_half) {
+                    if (dist2_to<D>(xq, Xs, n, run_start[rr], x2) <= g.thr_half) {
 #pragma unroll
                         for (int l = 0; l < D; l++) {
                             float nd = x2[l] - xp[l];
@@ -780,14 +762,15 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
                        hipStream_t s) {
     DebugOut d = dbg ? *dbg : DebugOut{nullptr, nullptr, nullptr, nullptr};
     const int qb = queue_blocks(g);
+    int *rdbg = (d.N || d.half || d.border || d.synced) ? b.rdbg : nullptr;
 #define UPD_CASE(DD)                                                                          \
     case DD:                                                                                  \
-        k_update_tile<DD><<<qb, kBlock, 0, s>>>(b.Xs, b.X, b.cell_start, b.cell_g, b.Ts, b.Tc, \
-                                                b.ob, b.oe, b.nonuni, b.tile_off, b.cres,     \
-                                                b.cdbg, g, st, d);                            \
-        k_scatter_uniform<DD><<<grid_for(g.n), kBlock, 0, s>>>(                               \
-            b.point_cell, b.cell_start, b.nonuni, b.cres, b.cdbg, b.perm_s, b.X, b.perm, g.n, \
-            st, d);                                                                           \
+        k_update_tile<DD><<<qb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc,     \
+                                                b.ob, b.oe, b.run_start, b.cell_run,          \
+                                                b.tile_off, b.rres, rdbg, g, st);             \
+        k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.head, b.hpos, b.rres, rdbg,     \
+                                                            b.perm_s, b.X, b.perm, g.n, st,   \
+                                                            d);                               \
         break;
     switch (g.d) {
         UPD_CASE(1) UPD_CASE(2) UPD_CASE(3) UPD_CASE(4)
@@ -800,11 +783,11 @@ void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s) 
     const int qb = queue_blocks(g);
 #define EX_CASE(DD)                                                                           \
     case DD:                                                                                  \
-        k_extra_tile<DD><<<qb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.ob, b.oe,      \
-                                               b.nonuni, b.tile_off, b.pairs, b.pair_cap, g,  \
+        k_extra_tile<DD><<<qb, kBlock, 0, s>>>(b.Xs, b.cell_g, b.ob, b.oe, b.run_start,       \
+                                               b.cell_run, b.tile_off, b.pairs, b.pair_cap, g, \
                                                st);                                           \
-        k_extra_mbr<DD><<<1024, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.ob, b.oe,     \
-                                                b.nonuni, b.pairs, b.pair_cap, g, st);        \
+        k_extra_mbr<DD><<<1024, kBlock, 0, s>>>(b.Xs, b.cell_g, b.ob, b.oe, b.run_start,      \
+                                                b.cell_run, b.pairs, b.pair_cap, g, st);      \
         break;
     switch (g.d) {
         EX_CASE(1) EX_CASE(2) EX_CASE(3) EX_CASE(4)
