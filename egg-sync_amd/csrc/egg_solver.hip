@@ -238,6 +238,30 @@ __global__ void k_soa_to_rows(const int *__restrict__ soa, int n, int d, const D
         for (int l = 0; l < d; l++) rows[c * d + l] = soa[(long long)l * n + c];
 }
 
+// Learned-jump features of every point, point-id order, 4d floats each:
+// position, own-cell mean sin and mean cos (the cell totals of the summary
+// pass divided by the count), displacement since `prev` divided by eps.
+__global__ void k_features(const float *__restrict__ Xs, const int *__restrict__ perm_s,
+                           const int *__restrict__ point_cell, const int *__restrict__ cell_start,
+                           const float *__restrict__ Ts, const float *__restrict__ Tc,
+                           const float *__restrict__ prev, int n, int d, float inv_eps,
+                           float *__restrict__ feat) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long p = perm_s[i];
+        const int c = point_cell[i];
+        const float inv_cnt = 1.f / (float)(cell_start[c + 1] - cell_start[c]);
+        float *f = feat + p * 4 * d;
+        for (int l = 0; l < d; l++) {
+            const float x = Xs[(long long)l * n + i];
+            f[l] = x;
+            f[d + l] = Ts[(long long)l * n + c] * inv_cnt;
+            f[2 * d + l] = Tc[(long long)l * n + c] * inv_cnt;
+            f[3 * d + l] = prev ? (x - prev[p * d + l]) * inv_eps : 0.f;
+        }
+    }
+}
+
 __global__ void k_snapshot_flag(DevState *st) {
     if (st->done) return;
     st->flag_local = st->flag;
@@ -597,6 +621,25 @@ int egg_sync_iteration_device(const float *d_D, int64_t n, int32_t d, float eps,
     HIP_TRY(hipGetLastError());
     if (flag_local) *flag_local = hs.flag_local;
     if (flag_final) *flag_final = hs.flag;
+    return EGG_SYNC_OK;
+}
+
+int egg_sync_features_device(const float *d_D, const float *d_prev, int64_t n, int32_t d,
+                             float eps, float *d_feat, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if (!d_feat) return EGG_SYNC_E_ARG;
+    Geo g;
+    Layout L;
+    hipStream_t s;
+    int rc = setup_single(d_D, n, d, eps, nullptr, g, L, s, stream);
+    if (rc) return rc;
+    build_grid(g, L.b, nullptr, L.st, s);
+    cell_summaries(g, L.b, nullptr, L.st, s);
+    k_features<<<grid_blocks(g.n), kBlock, 0, s>>>(L.b.Xs, L.b.perm_s, L.b.point_cell,
+                                                   L.b.cell_start, L.b.Ts, L.b.Tc, d_prev, g.n, d,
+                                                   1.f / eps, d_feat);
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipGetLastError());
     return EGG_SYNC_OK;
 }
 

@@ -22,6 +22,8 @@ EXPORTED = [
     "egg_sync_default_options", "egg_sync_solve", "egg_sync_solve_ex", "egg_sync_solve_device",
     "egg_sync_build_grid_device", "egg_sync_iteration_device", "egg_sync_rc_device",
     "egg_sync_geometry", "egg_sync_release", "egg_sync_error_string",
+    "egg_sync_features_device", "egg_sync_jump_device", "egg_mlp_param_count",
+    "egg_mlp_workspace_floats", "egg_mlp_forward_device", "egg_mlp_train_step_device",
 ]
 
 
@@ -77,6 +79,14 @@ def lib():
         L.egg_sync_geometry.argtypes = [i64, i32, f32, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_float)]
         L.egg_sync_release.argtypes = []
+        L.egg_sync_features_device.argtypes = [P, P, i64, i32, f32, P, P]
+        L.egg_sync_jump_device.argtypes = [P, P, i64, i32, f32, P]
+        L.egg_mlp_param_count.argtypes = [i32, i32, i32, ctypes.POINTER(ctypes.c_int64)]
+        L.egg_mlp_workspace_floats.argtypes = [i32, i32, i32, i64]
+        L.egg_mlp_workspace_floats.restype = ctypes.c_size_t
+        L.egg_mlp_forward_device.argtypes = [P, i32, i32, i32, P, i64, P, P]
+        L.egg_mlp_train_step_device.argtypes = [P, P, P, i32, i32, i32, P, P, i64, f32, i32, P,
+                                                i64, P, ctypes.POINTER(ctypes.c_double), P]
         L.egg_sync_error_string.argtypes = [ctypes.c_int]
         L.egg_sync_error_string.restype = ctypes.c_char_p
         _lib = L

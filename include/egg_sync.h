@@ -25,6 +25,7 @@
 #ifndef EGG_SYNC_H
 #define EGG_SYNC_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -123,6 +124,42 @@ int egg_sync_iteration_device(const float *d_D, int64_t n, int32_t d, float eps,
  * FP64 sums (written to *rc_out on the host). */
 int egg_sync_rc_device(const float *d_D, int64_t n, int32_t d, float eps, double *rc_out,
                        void *stream);
+
+/* ---- Learned jump (no counterpart in the reference) ------------------------
+ * A small MLP predicts, per point, a multi-iteration jump toward its attractor
+ * from 4d features; the plain iteration then resumes and its labels decide.
+ * Parameters are one flat float array W1[h][in], b1[h], W2[h][h], b2[h],
+ * W3[out][h], b3[out]. */
+
+/* Features of device positions d_D (n x d, point-id order): per point, 4d
+ * floats = position, own-cell mean sin, own-cell mean cos (cell totals of the
+ * summary pass / count), (x - prev) / eps (0 if d_prev is NULL). */
+int egg_sync_features_device(const float *d_D, const float *d_prev, int64_t n, int32_t d,
+                             float eps, float *d_feat, void *stream);
+
+/* d_X <- clamp(d_X + scale * d_Y, 0, 1), n x d. */
+int egg_sync_jump_device(float *d_X, const float *d_Y, int64_t n, int32_t d, float scale,
+                         void *stream);
+
+/* Number of parameters of an in-hidden-hidden-out MLP. */
+int egg_mlp_param_count(int32_t in, int32_t hidden, int32_t out, int64_t *count);
+
+/* Device workspace (floats) a training step on `batch` samples needs. */
+size_t egg_mlp_workspace_floats(int32_t in, int32_t hidden, int32_t out, int64_t batch);
+
+/* Inference: d_Y[batch x out] = MLP(d_X[batch x in]); fused kernel for 12-128-3. */
+int egg_mlp_forward_device(const float *d_params, int32_t in, int32_t hidden, int32_t out,
+                           const float *d_X, int64_t batch, float *d_Y, void *stream);
+
+/* One training step, loss mean_b sum_o (MLP(x_b) - t_b)^2: gradients into
+ * d_grad (same layout as the parameters); if lr > 0 an Adam step (number
+ * `step`, 1-based, moments d_adam_m / d_adam_v) updates d_params.  *loss_out
+ * (host, may be NULL) receives the loss before the update. */
+int egg_mlp_train_step_device(float *d_params, float *d_adam_m, float *d_adam_v, int32_t in,
+                              int32_t hidden, int32_t out, const float *d_X, const float *d_T,
+                              int64_t batch, float lr, int32_t step, float *d_work,
+                              int64_t work_floats, float *d_grad, double *loss_out,
+                              void *stream);
 
 /* Grid geometry the solver derives from (n, d, eps) (EGG_SynC.cu:1249-1260):
  * out[0]=w, out[1]=od, out[2]=O, out[3]=R, out[4]=key bits; *cs_out cell side. */

@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "egg_sync.h")
 
 def _declared_functions():
     src = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(egg_sync_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(egg_(?:sync|mlp)_\w+)\s*\(", src, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
