@@ -88,6 +88,20 @@ def build_grid(X, eps: float, order_in=None):
                 oe=oe.cpu().numpy(), C=C)
 
 
+def iteration_device(D: torch.Tensor, eps: float, order_in: torch.Tensor = None):
+    """One EGG iteration, device in / device out: (next positions, visiting order)."""
+    D = D.contiguous()
+    n, d = D.shape
+    Xn = torch.empty_like(D)
+    order = torch.empty(n, dtype=torch.int32, device=D.device)
+    fl, ff = ctypes.c_int32(), ctypes.c_int32()
+    nat.check(nat.lib().egg_sync_iteration_device(nat.ptr(D), n, d, float(eps), nat.ptr(order_in),
+                                                  nat.ptr(Xn), nat.ptr(order), None, None, None,
+                                                  None, ctypes.byref(fl), ctypes.byref(ff),
+                                                  _stream()), "egg_sync_iteration_device")
+    return Xn, order
+
+
 def iteration(X, eps: float, order_in=None):
     """One EGG iteration on the GPU; same outputs as the CPU oracle's egg_iteration."""
     D = _dev_f32(X)

@@ -293,3 +293,24 @@ def test_rc_of_collapsed_clusters(O):
     X2 = np.array([[0.5, 0.5], [0.5, 0.53]], f32)
     dist = float(np.sqrt(_dist2_f32(X2[1:], X2[0])[0]))
     assert abs(O.rc(X2, 0.05) - (1.0 + float(np.exp(f32(-dist)))) / 2.0) < 1e-7
+
+
+# ------------------------------------------------- learned-jump MLP oracle ----
+
+def test_mlp_oracle_gradient_finite_differences():
+    import mlp_oracle as M
+    rng = np.random.default_rng(0)
+    n_in, h, n_out, B = 5, 7, 3, 11
+    P = rng.normal(0, 0.5, sum(int(np.prod(s)) for _, s in M.sizes(n_in, h, n_out)))
+    X = rng.normal(0, 1, (B, n_in))
+    T = rng.normal(0, 1, (B, n_out))
+    loss, g = M.loss_and_grad(P, X, T, n_in, h, n_out)
+    num = np.empty_like(P)
+    for k in range(P.size):
+        e = np.zeros_like(P)
+        e[k] = 1e-6
+        num[k] = (M.loss_and_grad(P + e, X, T, n_in, h, n_out)[0] -
+                  M.loss_and_grad(P - e, X, T, n_in, h, n_out)[0]) / 2e-6
+    assert np.abs(num - g).max() <= 1e-6 * max(1.0, np.abs(g).max())
+    O, _ = M.forward(P, X, n_in, h, n_out)
+    assert abs(loss - np.sum((O - T) ** 2) / B) < 1e-12
