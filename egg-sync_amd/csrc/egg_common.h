@@ -49,6 +49,8 @@ struct Geo {
     int key_bits;      // total key bits
     int kw;            // 64-bit key words (1 or 2)
     int wpow[kMaxD];   // w^l, l < od
+    int qblocks;       // workgroups of the tile-queue kernels (0 = automatic)
+    int qthreads;      // threads per workgroup of those kernels (64..256, multiple of 64)
 };
 
 // Device-resident loop state (one per solve).

@@ -34,9 +34,17 @@ struct Workspace {
     size_t bytes = 0;
     hipStream_t stream = nullptr;
     DevState *h_state = nullptr;  // pinned readback
+    hipGraphExec_t gexec = nullptr;  // captured loop body, reused while gkey matches
+    std::vector<char> gkey;
 };
 
 Workspace g_ws[64];
+
+void drop_graph(hipGraphExec_t &ge, std::vector<char> &key) {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    ge = nullptr;
+    key.clear();
+}
 
 size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -203,6 +211,7 @@ int acquire(const Geo &g, Layout &L, Workspace *&ws) {
     ws = &g_ws[dev];
     Layout probe = carve(nullptr, g);
     if (ws->bytes < probe.bytes) {
+        drop_graph(ws->gexec, ws->gkey);
         if (ws->base) HIP_TRY(hipFree(ws->base));
         ws->base = nullptr;
         ws->bytes = 0;
@@ -306,7 +315,23 @@ void default_options(egg_sync_options &o) {
     o.timing = 0;
     o.iters_per_sync = 4;
     o.validate = 1;
+    o.use_graph = 0;
+    o.queue_blocks = 0;
+    o.queue_threads = 0;
+    o.streams = 0;
 }
+
+void apply_tuning(const egg_sync_options &opt, Geo &g) {
+    g.qblocks = std::max(0, opt.queue_blocks);
+    g.qthreads = opt.queue_threads;
+}
+
+template <typename T>
+void key_add(std::vector<char> &k, const T &v) {
+    const char *p = (const char *)&v;
+    k.insert(k.end(), p, p + sizeof(T));
+}
+
 
 struct Events {
     std::vector<hipEvent_t> ev;
@@ -356,6 +381,30 @@ void enqueue_iteration(const Geo &g, Layout &L, const egg_sync_options &opt, hip
     if (ev) (void)hipEventRecord(ev[3], s);
 }
 
+// Records `K` loop iterations enqueued by `body` on the private stream `cap`
+// into an executable graph.  The loop body has no memset or host-copy nodes:
+// every kernel reads the device state and returns at once after convergence,
+// so replaying the graph past the last iteration changes nothing.
+template <typename Body>
+int capture_graph(hipStream_t cap, hipGraphExec_t &ge, Body &&body) {
+    HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    body();
+    hipGraph_t graph = nullptr;
+    const hipError_t e1 = hipStreamEndCapture(cap, &graph);
+    const hipError_t e0 = hipGetLastError();
+    if (e1 != hipSuccess || e0 != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_err(e1 != hipSuccess ? e1 : (e0 != hipSuccess ? e0 : hipErrorUnknown));
+    }
+    const hipError_t e2 = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) {
+        ge = nullptr;
+        return hip_err(e2);
+    }
+    return 0;
+}
+
 int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
                       const egg_sync_options &opt, int32_t *d_labels, float *d_final,
                       egg_sync_stats *so, hipStream_t s_user, const float *h_D,
@@ -385,6 +434,7 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
     Geo g;
     int rc = compute_geo(n64, d, eps, g);
     if (rc) return rc;
+    apply_tuning(opt, g);
     const int n = g.n;
     Layout L;
     Workspace *ws = nullptr;
@@ -418,14 +468,41 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
     }
     double st_build = 0, st_upd = 0, st_extra = 0;
     int done_iters = 0;
+    if (opt.use_graph) {
+        std::vector<char> key;
+        key_add(key, ws->base);
+        key_add(key, g);
+        key_add(key, K);
+        key_add(key, opt.stop_rule);
+        if (!ws->gexec || key != ws->gkey) {
+            drop_graph(ws->gexec, ws->gkey);
+            rc = capture_graph(ws->stream, ws->gexec, [&] {
+                for (int k = 0; k < K; k++) enqueue_iteration(g, L, opt, ws->stream, nullptr);
+            });
+            if (rc) return rc;
+            ws->gkey = key;
+        }
+    }
     for (;;) {
-        for (int k = 0; k < K; k++)
-            enqueue_iteration(g, L, opt, s, opt.timing ? &E.ev[4 * k] : nullptr);
+        if (opt.use_graph) {
+            if (opt.timing) (void)hipEventRecord(E.ev[0], s);
+            HIP_TRY(hipGraphLaunch(ws->gexec, s));
+            if (opt.timing) (void)hipEventRecord(E.ev[1], s);
+        } else {
+            for (int k = 0; k < K; k++)
+                enqueue_iteration(g, L, opt, s, opt.timing ? &E.ev[4 * k] : nullptr);
+        }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(ws->h_state, L.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         const int real = ws->h_state->iter - done_iters;
-        if (opt.timing) {
+        if (opt.timing && opt.use_graph) {
+            const double t = ms_between(E.ev[0], E.ev[1]);
+            st_upd += t;
+            for (int k = 0; k < real && k < K; k++)
+                if (so && so->itr_ms && done_iters + k < so->itr_ms_cap)
+                    so->itr_ms[done_iters + k] = (float)(t / std::max(real, 1));
+        } else if (opt.timing) {
             for (int k = 0; k < real && k < K; k++) {
                 hipEvent_t *ev = &E.ev[4 * k];
                 const double b = ms_between(ev[0], ev[1]);
@@ -518,6 +595,317 @@ int setup_single(const float *d_D, int64_t n64, int d, float eps, const int32_t 
     return 0;
 }
 
+// ------------------------------------------------------------ batched solve --
+//
+// Independent solves share one device arena (instance k's buffers at its own
+// offset, carved exactly as for a single solve) and one array of loop states.
+// Instances are dealt to HIP streams, largest first onto the least-loaded
+// stream; a stream runs its instances' iterations back to back, so small
+// solves, which cannot fill the chip alone, overlap with each other and with
+// the large ones.  Every `iters_per_sync` iterations one copy brings all loop
+// states to the host, and the loop ends when every instance is done (the
+// kernels of a finished instance return at once).  With use_graph the
+// iterations of all streams are captured once, forked from and joined to one
+// origin stream, and the graph is replayed.
+
+struct BatchWs {
+    void *base = nullptr;
+    size_t bytes = 0;
+    DevState *states = nullptr;
+    DevState *h_states = nullptr;
+    int states_cap = 0;
+    int *err = nullptr;
+    hipStream_t origin = nullptr;  // used when the caller passes no stream; capture origin
+    std::vector<hipStream_t> streams;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<char> gkey;
+};
+
+BatchWs g_bws[64];
+
+constexpr int kMaxStreams = 16;
+
+int batch_resources(BatchWs &bw, size_t bytes, int n_inst, int S) {
+    if (bw.bytes < bytes) {
+        drop_graph(bw.gexec, bw.gkey);
+        if (bw.base) HIP_TRY(hipFree(bw.base));
+        bw.base = nullptr;
+        bw.bytes = 0;
+        const size_t want = bytes + bytes / 8;
+        if (hipMalloc(&bw.base, want) != hipSuccess) {
+            (void)hipGetLastError();
+            return EGG_SYNC_E_ALLOC;
+        }
+        bw.bytes = want;
+    }
+    if (bw.states_cap < n_inst) {
+        drop_graph(bw.gexec, bw.gkey);
+        if (bw.states) HIP_TRY(hipFree(bw.states));
+        if (bw.h_states) HIP_TRY(hipHostFree(bw.h_states));
+        bw.states = nullptr;
+        bw.h_states = nullptr;
+        bw.states_cap = 0;
+        if (hipMalloc((void **)&bw.states, sizeof(DevState) * n_inst) != hipSuccess) {
+            (void)hipGetLastError();
+            return EGG_SYNC_E_ALLOC;
+        }
+        HIP_TRY(hipHostMalloc((void **)&bw.h_states, sizeof(DevState) * n_inst, 0));
+        bw.states_cap = n_inst;
+    }
+    if (!bw.err) HIP_TRY(hipMalloc((void **)&bw.err, sizeof(int)));
+    if (!bw.origin) HIP_TRY(hipStreamCreate(&bw.origin));
+    while ((int)bw.streams.size() < S) {
+        hipStream_t t;
+        HIP_TRY(hipStreamCreate(&t));
+        bw.streams.push_back(t);
+    }
+    if (!bw.fork) HIP_TRY(hipEventCreateWithFlags(&bw.fork, hipEventDisableTiming));
+    while ((int)bw.joins.size() < S) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        bw.joins.push_back(e);
+    }
+    return 0;
+}
+
+void release_batch(BatchWs &bw) {
+    drop_graph(bw.gexec, bw.gkey);
+    if (bw.base) (void)hipFree(bw.base);
+    if (bw.states) (void)hipFree(bw.states);
+    if (bw.h_states) (void)hipHostFree(bw.h_states);
+    if (bw.err) (void)hipFree(bw.err);
+    for (auto t : bw.streams) (void)hipStreamDestroy(t);
+    if (bw.origin) (void)hipStreamDestroy(bw.origin);
+    if (bw.fork) (void)hipEventDestroy(bw.fork);
+    for (auto e : bw.joins) (void)hipEventDestroy(e);
+    bw = BatchWs{};
+}
+
+int fork_streams(BatchWs &bw, hipStream_t origin, int S) {
+    HIP_TRY(hipEventRecord(bw.fork, origin));
+    for (int j = 0; j < S; j++) HIP_TRY(hipStreamWaitEvent(bw.streams[j], bw.fork, 0));
+    return 0;
+}
+
+int join_streams(BatchWs &bw, hipStream_t origin, int S) {
+    for (int j = 0; j < S; j++) {
+        HIP_TRY(hipEventRecord(bw.joins[j], bw.streams[j]));
+        HIP_TRY(hipStreamWaitEvent(origin, bw.joins[j], 0));
+    }
+    return 0;
+}
+
+int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst,
+                     const egg_sync_options &opt, int32_t *d_labels, float *d_final,
+                     egg_sync_instance_result *res, egg_sync_stats *so, hipStream_t s_user) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    if (n_inst < 0 || (n_inst > 0 && !inst)) return EGG_SYNC_E_ARG;
+    if (opt.stop_rule != EGG_SYNC_STOP_EGG && opt.stop_rule != EGG_SYNC_STOP_RC)
+        return EGG_SYNC_E_ARG;
+    if (opt.label_rule != EGG_SYNC_LABELS_CELLS && opt.label_rule != EGG_SYNC_LABELS_SYNC)
+        return EGG_SYNC_E_ARG;
+    if (so) {
+        so->iterations = 0;
+        so->status = 0;
+        so->n_clusters = 0;
+        so->n_cells = 0;
+        so->rc = std::nan("");
+        for (int k = 0; k < 6; k++) so->stage_ms[k] = 0.0;
+        so->update_kernel_ms = 0.0;
+        so->update_launches = 0;
+        so->workspace_bytes = 0;
+    }
+    if (res)
+        for (int k = 0; k < n_inst; k++) res[k] = egg_sync_instance_result{0, 0, 0, 0};
+    std::vector<Geo> G(n_inst);
+    std::vector<size_t> off(n_inst, 0);
+    std::vector<int64_t> row_off(n_inst, 0), fl_off(n_inst, 0);
+    std::vector<int> active;
+    size_t total = 0;
+    int64_t rows = 0, fl = 0;
+    for (int k = 0; k < n_inst; k++) {
+        const egg_sync_instance &I = inst[k];
+        if (I.n < 0 || I.data_offset < 0) return EGG_SYNC_E_ARG;
+        row_off[k] = rows;
+        fl_off[k] = fl;
+        if (I.n == 0) continue;
+        const int rc = compute_geo(I.n, I.d, I.eps, G[k]);
+        if (rc) return rc;
+        apply_tuning(opt, G[k]);
+        off[k] = total;
+        total += carve(nullptr, G[k]).bytes;
+        rows += I.n;
+        fl += I.n * I.d;
+        active.push_back(k);
+    }
+    if (active.empty()) {
+        if (so) so->total_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        return EGG_SYNC_OK;
+    }
+    if (!d_D || !d_labels) return EGG_SYNC_E_ARG;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return EGG_SYNC_E_NODEV;
+    BatchWs &bw = g_bws[dev];
+    int S = opt.streams > 0 ? std::min(opt.streams, kMaxStreams) : 4;
+    S = std::max(1, std::min(S, (int)active.size()));
+    int rc = batch_resources(bw, total, n_inst, S);
+    if (rc) return rc;
+    std::vector<Layout> Ls(n_inst);
+    for (int k : active) {
+        Ls[k] = carve((char *)bw.base + off[k], G[k]);
+        Ls[k].st = bw.states + k;
+    }
+    hipStream_t s = s_user ? s_user : bw.origin;
+
+    if (opt.validate) {
+        HIP_TRY(hipMemsetAsync(bw.err, 0, sizeof(int), s));
+        for (int k : active)
+            launch_validate(d_D + inst[k].data_offset, inst[k].n * inst[k].d, bw.err, s);
+        int herr = 0;
+        HIP_TRY(hipMemcpyAsync(&herr, bw.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (herr) return EGG_SYNC_E_RANGE;
+    }
+    for (int k = 0; k < n_inst; k++) {
+        init_state(bw.h_states[k], opt);
+        if (inst[k].n == 0) bw.h_states[k].done = 1;
+    }
+    HIP_TRY(hipMemcpyAsync(bw.states, bw.h_states, sizeof(DevState) * n_inst,
+                           hipMemcpyHostToDevice, s));
+    for (int k : active)
+        launch_aos_to_soa(d_D + inst[k].data_offset, Ls[k].b.X, Ls[k].b.perm, G[k].n, G[k].d, s);
+
+    // largest first onto the least-loaded stream
+    std::vector<int> order = active;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return inst[a].n * inst[a].d > inst[b].n * inst[b].d; });
+    std::vector<std::vector<int>> per(S);
+    std::vector<int64_t> load(S, 0);
+    for (int k : order) {
+        const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        per[j].push_back(k);
+        load[j] += inst[k].n * inst[k].d;
+    }
+    const int K = std::max(1, opt.iters_per_sync);
+    auto enqueue_batch = [&](hipStream_t origin) -> int {
+        int r = fork_streams(bw, origin, S);
+        if (r) return r;
+        for (int j = 0; j < S; j++)
+            for (int k : per[j])
+                for (int it = 0; it < K; it++)
+                    enqueue_iteration(G[k], Ls[k], opt, bw.streams[j], nullptr);
+        return join_streams(bw, origin, S);
+    };
+    if (opt.use_graph) {
+        std::vector<char> key;
+        key_add(key, bw.base);
+        key_add(key, bw.states);
+        key_add(key, K);
+        key_add(key, S);
+        key_add(key, opt.stop_rule);
+        for (int j = 0; j < S; j++)
+            for (int k : per[j]) {
+                key_add(key, k);
+                key_add(key, off[k]);
+                key_add(key, G[k]);
+            }
+        if (!bw.gexec || key != bw.gkey) {
+            drop_graph(bw.gexec, bw.gkey);
+            int inner = 0;
+            rc = capture_graph(bw.origin, bw.gexec, [&] { inner = enqueue_batch(bw.origin); });
+            if (rc) return rc;
+            if (inner) {
+                drop_graph(bw.gexec, bw.gkey);
+                return inner;
+            }
+            bw.gkey = key;
+        }
+    }
+    Events E;
+    if (opt.timing) {
+        rc = E.ensure(4);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(E.ev[0], s));
+    }
+    const auto t_setup = clk::now();
+    int max_iter_seen = 0;
+    for (;;) {
+        if (opt.use_graph) {
+            HIP_TRY(hipGraphLaunch(bw.gexec, s));
+        } else {
+            rc = enqueue_batch(s);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(bw.h_states, bw.states, sizeof(DevState) * n_inst,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        bool all = true;
+        for (int k : active) {
+            all = all && bw.h_states[k].done;
+            max_iter_seen = std::max(max_iter_seen, bw.h_states[k].iter);
+        }
+        if (all) break;
+    }
+    if (opt.timing) HIP_TRY(hipEventRecord(E.ev[1], s));
+    // final grids and labels (EGG_SynC.cu:1504-1541), one per instance
+    rc = fork_streams(bw, s, S);
+    if (rc) return rc;
+    for (int j = 0; j < S; j++) {
+        hipStream_t sj = bw.streams[j];
+        for (int k : per[j]) {
+            Layout &L = Ls[k];
+            build_grid(G[k], L.b, nullptr, L.st, sj);
+            int32_t *lab = d_labels + row_off[k];
+            if (opt.label_rule == EGG_SYNC_LABELS_CELLS)
+                launch_labels_egg(G[k], L.b, lab, L.b.nchunk, L.b.wi_off, L.st, sj);
+            else
+                launch_labels_sync(G[k], L.b, lab, L.b.head, L.st, sj);
+            if (opt.compute_rc && opt.stop_rule == EGG_SYNC_STOP_EGG)
+                launch_update_sync(G[k], L.b, L.st, L.rc_part, false, sj);
+            if (d_final)
+                launch_soa_to_aos_by_perm(L.b.Xs, L.b.perm_s, d_final + fl_off[k], G[k].n, G[k].d,
+                                          sj);
+        }
+    }
+    rc = join_streams(bw, s, S);
+    if (rc) return rc;
+    if (opt.timing) HIP_TRY(hipEventRecord(E.ev[2], s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(bw.h_states, bw.states, sizeof(DevState) * n_inst,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    bool capped = false;
+    int64_t clusters = 0, cells = 0;
+    for (int k : active) {
+        const DevState &f = bw.h_states[k];
+        capped = capped || f.capped;
+        clusters += f.n_clusters;
+        cells += f.C;
+        if (res)
+            res[k] = egg_sync_instance_result{f.iter, f.capped ? EGG_SYNC_CAPPED : EGG_SYNC_OK,
+                                              f.n_clusters, f.C};
+    }
+    if (so) {
+        so->iterations = max_iter_seen;
+        so->status = capped ? EGG_SYNC_CAPPED : EGG_SYNC_OK;
+        so->n_clusters = (int32_t)std::min<int64_t>(clusters, INT_MAX);
+        so->n_cells = (int32_t)std::min<int64_t>(cells, INT_MAX);
+        so->stage_ms[0] = std::chrono::duration<double, std::milli>(t_setup - t0).count();
+        if (opt.timing) {
+            so->stage_ms[2] = ms_between(E.ev[0], E.ev[1]);
+            so->stage_ms[4] = ms_between(E.ev[1], E.ev[2]);
+        }
+        so->workspace_bytes = (int64_t)(total + sizeof(DevState) * n_inst);
+        so->total_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    }
+    return capped ? EGG_SYNC_CAPPED : EGG_SYNC_OK;
+}
+
 }  // namespace
 }  // namespace egg
 
@@ -562,6 +950,19 @@ int egg_sync_solve_device(const float *d_D, int64_t n, int32_t d, float eps,
     std::lock_guard<std::mutex> lk(g_mutex);
     return solve_device_impl(d_D, n, d, eps, o, d_labels, d_final, stats_out,
                              (hipStream_t)stream, nullptr, nullptr, nullptr);
+}
+
+int egg_sync_solve_batch_device(const float *d_D, const egg_sync_instance *inst,
+                                int32_t n_inst, const egg_sync_options *opt,
+                                int32_t *d_labels, float *d_final,
+                                egg_sync_instance_result *results, egg_sync_stats *stats,
+                                void *stream) {
+    egg_sync_options o;
+    if (opt) o = *opt;
+    else default_options(o);
+    std::lock_guard<std::mutex> lk(g_mutex);
+    return solve_batch_impl(d_D, inst, n_inst, o, d_labels, d_final, results, stats,
+                            (hipStream_t)stream);
 }
 
 int egg_sync_build_grid_device(const float *d_D, int64_t n, int32_t d, float eps,
@@ -682,10 +1083,12 @@ int egg_sync_release(void) {
     if (hipGetDevice(&dev) != hipSuccess) return EGG_SYNC_E_NODEV;
     if (dev < 0 || dev >= 64) return EGG_SYNC_E_NODEV;
     Workspace &ws = g_ws[dev];
+    drop_graph(ws.gexec, ws.gkey);
     if (ws.base) (void)hipFree(ws.base);
     if (ws.stream) (void)hipStreamDestroy(ws.stream);
     if (ws.h_state) (void)hipHostFree(ws.h_state);
     ws = Workspace{};
+    release_batch(g_bws[dev]);
     return EGG_SYNC_OK;
 }
 

@@ -753,19 +753,26 @@ static inline int grid_for(long long n) {
 // Enough workgroups to fill the chip; they drain the tile queue and exit.
 constexpr int kQueueBlocks = 2048;
 
+static inline int queue_threads(const Geo &g) {
+    return g.qthreads >= kWave && g.qthreads <= kBlock && g.qthreads % kWave == 0 ? g.qthreads
+                                                                                 : kBlock;
+}
+
 static inline int queue_blocks(const Geo &g) {
     const long long tiles_max = (long long)g.n / kTile + g.n;  // >= n/64 + C
-    return (int)std::min<long long>(kQueueBlocks, (tiles_max + 3) / 4);
+    const int waves = queue_threads(g) / kWave;
+    const long long cap = g.qblocks > 0 ? g.qblocks : kQueueBlocks * (kBlock / kWave) / waves;
+    return (int)std::max<long long>(1, std::min<long long>(cap, (tiles_max + waves - 1) / waves));
 }
 
 void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *dbg,
                        hipStream_t s) {
     DebugOut d = dbg ? *dbg : DebugOut{nullptr, nullptr, nullptr, nullptr};
-    const int qb = queue_blocks(g);
+    const int qb = queue_blocks(g), qt = queue_threads(g);
     int *rdbg = (d.N || d.half || d.border || d.synced) ? b.rdbg : nullptr;
 #define UPD_CASE(DD)                                                                          \
     case DD:                                                                                  \
-        k_update_tile<DD><<<qb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc,     \
+        k_update_tile<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc,     \
                                                 b.ob, b.oe, b.run_start, b.cell_run,          \
                                                 b.tile_off, b.rres, rdbg, g, st);             \
         k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.head, b.hpos, b.rres, rdbg,     \
@@ -780,10 +787,10 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
 }
 
 void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s) {
-    const int qb = queue_blocks(g);
+    const int qb = queue_blocks(g), qt = queue_threads(g);
 #define EX_CASE(DD)                                                                           \
     case DD:                                                                                  \
-        k_extra_tile<DD><<<qb, kBlock, 0, s>>>(b.Xs, b.cell_g, b.ob, b.oe, b.run_start,       \
+        k_extra_tile<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_g, b.ob, b.oe, b.run_start,       \
                                                b.cell_run, b.tile_off, b.pairs, b.pair_cap, g, \
                                                st);                                           \
         k_extra_mbr<DD><<<1024, kBlock, 0, s>>>(b.Xs, b.cell_g, b.ob, b.oe, b.run_start,      \

@@ -24,6 +24,7 @@ EXPORTED = [
     "egg_sync_geometry", "egg_sync_release", "egg_sync_error_string",
     "egg_sync_features_device", "egg_sync_jump_device", "egg_mlp_param_count",
     "egg_mlp_workspace_floats", "egg_mlp_forward_device", "egg_mlp_train_step_device",
+    "egg_sync_solve_batch_device",
 ]
 
 
@@ -31,7 +32,19 @@ class Options(ctypes.Structure):
     _fields_ = [("stop_rule", ctypes.c_int32), ("label_rule", ctypes.c_int32),
                 ("lam", ctypes.c_float), ("max_iter", ctypes.c_int32),
                 ("compute_rc", ctypes.c_int32), ("timing", ctypes.c_int32),
-                ("iters_per_sync", ctypes.c_int32), ("validate", ctypes.c_int32)]
+                ("iters_per_sync", ctypes.c_int32), ("validate", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("queue_blocks", ctypes.c_int32),
+                ("queue_threads", ctypes.c_int32), ("streams", ctypes.c_int32)]
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [("data_offset", ctypes.c_int64), ("n", ctypes.c_int64), ("d", ctypes.c_int32),
+                ("eps", ctypes.c_float)]
+
+
+class InstanceResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("n_clusters", ctypes.c_int32), ("n_cells", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -87,6 +100,10 @@ def lib():
         L.egg_mlp_forward_device.argtypes = [P, i32, i32, i32, P, i64, P, P]
         L.egg_mlp_train_step_device.argtypes = [P, P, P, i32, i32, i32, P, P, i64, f32, i32, P,
                                                 i64, P, ctypes.POINTER(ctypes.c_double), P]
+        L.egg_sync_solve_batch_device.argtypes = [P, ctypes.POINTER(Instance), i32,
+                                                  ctypes.POINTER(Options), P, P,
+                                                  ctypes.POINTER(InstanceResult),
+                                                  ctypes.POINTER(Stats), P]
         L.egg_sync_error_string.argtypes = [ctypes.c_int]
         L.egg_sync_error_string.restype = ctypes.c_char_p
         _lib = L

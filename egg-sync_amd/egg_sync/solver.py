@@ -26,15 +26,20 @@ def _stream():
 
 def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-3, max_iter=0,
           compute_rc=False, timing=False, iters_per_sync=4, return_positions=True,
-          itr_cap=16384):
-    """Solve on the GPU.  Returns dict(labels int32 numpy, X final numpy or None, stats)."""
+          itr_cap=16384, use_graph=False, queue_blocks=0, queue_threads=0):
+    """Solve on the GPU.  Returns dict(labels int32 numpy, X final numpy or None, stats).
+
+    use_graph replays the loop body from a captured hipGraph; queue_blocks /
+    queue_threads set the launch shape of the tile-queue kernels (0 = automatic).
+    None of these change the arithmetic."""
     D = _dev_f32(X)
     n, d = D.shape
     if label_rule is None:
         label_rule = nat.LABELS_SYNC if stop_rule == nat.STOP_RC else nat.LABELS_CELLS
     opt = nat.default_options(stop_rule=stop_rule, label_rule=label_rule, lam=lam,
                               max_iter=max_iter, compute_rc=int(compute_rc), timing=int(timing),
-                              iters_per_sync=iters_per_sync)
+                              iters_per_sync=iters_per_sync, use_graph=int(use_graph),
+                              queue_blocks=queue_blocks, queue_threads=queue_threads)
     labels = torch.empty(n, dtype=torch.int32, device=D.device)
     final = torch.empty_like(D) if return_positions else None
     st = nat.Stats()
@@ -60,6 +65,76 @@ def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-
     stats["rc_trace"] = rc_tr[:it].copy()
     return dict(labels=labels.cpu().numpy(), X=None if final is None else final.cpu().numpy(),
                 stats=stats, code=rc)
+
+
+def pack_instances(datasets, eps_values=None):
+    """Packs datasets (arrays or tensors, n_k x d_k) into one device buffer.
+
+    With `eps_values` a list of lists, instance list = every dataset with each
+    of its eps values, sharing the dataset's rows; otherwise `datasets` is a
+    list of (X, eps) pairs.  Returns (D flat device tensor, [(offset, n, d, eps)])."""
+    if eps_values is None:
+        pairs = list(datasets)
+        datasets = [x for x, _ in pairs]
+        eps_values = [[e] for _, e in pairs]
+    flat, specs, off = [], [], 0
+    for X, es in zip(datasets, eps_values):
+        T = _dev_f32(X)
+        n, d = T.shape
+        flat.append(T.reshape(-1))
+        for e in es:
+            specs.append((off, n, d, float(e)))
+        off += n * d
+    D = torch.cat(flat) if flat else torch.empty(0, dtype=torch.float32, device="cuda")
+    return D, specs
+
+
+def solve_batch(D: torch.Tensor, specs, *, stop_rule=nat.STOP_EGG, label_rule=None,
+                lam=1 - 1e-3, max_iter=0, timing=False, iters_per_sync=4, use_graph=False,
+                streams=4, queue_blocks=0, queue_threads=0, return_positions=True):
+    """Many independent solves in one call (egg_sync_solve_batch_device).
+
+    `D` is a flat float32 device tensor, `specs` a list of (data_offset, n, d,
+    eps).  Returns dict(labels=[int32 numpy per instance], X=[final positions or
+    None], results=[dict per instance], stats)."""
+    if label_rule is None:
+        label_rule = nat.LABELS_SYNC if stop_rule == nat.STOP_RC else nat.LABELS_CELLS
+    opt = nat.default_options(stop_rule=stop_rule, label_rule=label_rule, lam=lam,
+                              max_iter=max_iter, timing=int(timing),
+                              iters_per_sync=iters_per_sync, use_graph=int(use_graph),
+                              queue_blocks=queue_blocks, queue_threads=queue_threads,
+                              streams=streams)
+    k = len(specs)
+    inst = (nat.Instance * max(k, 1))()
+    rows = fl = 0
+    for i, (o, n, d, e) in enumerate(specs):
+        inst[i].data_offset, inst[i].n, inst[i].d, inst[i].eps = int(o), int(n), int(d), float(e)
+        rows += int(n)
+        fl += int(n) * int(d)
+    labels = torch.empty(max(rows, 1), dtype=torch.int32, device=D.device)
+    final = torch.empty(max(fl, 1), dtype=torch.float32, device=D.device) \
+        if return_positions else None
+    res = (nat.InstanceResult * max(k, 1))()
+    st = nat.Stats()
+    rc = nat.lib().egg_sync_solve_batch_device(nat.ptr(D), inst, k, ctypes.byref(opt),
+                                               nat.ptr(labels), nat.ptr(final), res,
+                                               ctypes.byref(st), _stream())
+    nat.check(rc, "egg_sync_solve_batch_device")
+    lab = labels.cpu().numpy()
+    fin = final.cpu().numpy() if final is not None else None
+    out_l, out_x, out_r = [], [], []
+    r0 = f0 = 0
+    for i, (o, n, d, e) in enumerate(specs):
+        out_l.append(lab[r0:r0 + n].copy())
+        out_x.append(None if fin is None else fin[f0:f0 + n * d].reshape(n, d).copy())
+        out_r.append(dict(iterations=res[i].iterations, status=res[i].status,
+                          n_clusters=res[i].n_clusters, n_cells=res[i].n_cells))
+        r0 += n
+        f0 += n * d
+    stats = {k_: getattr(st, k_) for k_ in ("iterations", "status", "n_clusters", "n_cells",
+                                             "total_ms", "workspace_bytes")}
+    stats["stage_ms"] = list(st.stage_ms)
+    return dict(labels=out_l, X=out_x, results=out_r, stats=stats, code=rc)
 
 
 def build_grid(X, eps: float, order_in=None):

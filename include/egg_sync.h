@@ -18,6 +18,8 @@
  *   egg_sync_build_grid_device   build_the_grid (EGG_SynC.cu:248-341)
  *   egg_sync_iteration_device    one pass of the loop body (EGG_SynC.cu:1322-1491)
  *   egg_sync_rc_device           order parameter r_c (SynC.cpp:42-56, FP64 sums)
+ *   egg_sync_solve_batch_device  many EGG_SynC calls at once (run_experiment.py:294-331 loops
+ *                                over datasets and eps values one call at a time)
  * Threading: one solve at a time per process (the library caches one
  * workspace per device, like the reference's global allocation map,
  * GPU_utils.cu:176-178); calls are synchronous with respect to the host.
@@ -58,6 +60,13 @@ typedef struct egg_sync_options {
     int32_t timing;          /* 1: per-stage / per-iteration GPU event timings    */
     int32_t iters_per_sync;  /* iterations enqueued between host checks (>= 1)    */
     int32_t validate;        /* 1: check the input lies in [0,1] (default 1)      */
+    int32_t use_graph;       /* 1: capture iters_per_sync iterations in one hipGraph
+                                and replay it; with timing=1 the loop time is then
+                                reported as a whole in stage_ms[2]                */
+    int32_t queue_blocks;    /* workgroups of the tile-queue kernels (0 = auto)   */
+    int32_t queue_threads;   /* threads per such workgroup: 64, 128, 192 or 256
+                                (0 = 256)                                         */
+    int32_t streams;         /* batched solve: concurrent HIP streams (0 = 4)     */
 } egg_sync_options;
 
 typedef struct egg_sync_stats {
@@ -98,6 +107,34 @@ int egg_sync_solve_ex(const float *h_D, int64_t n, int32_t d, float eps,
 int egg_sync_solve_device(const float *d_D, int64_t n, int32_t d, float eps,
                           const egg_sync_options *opt, int32_t *d_labels, float *d_final,
                           egg_sync_stats *stats_out, void *stream);
+
+/* ---- Batched solve (no counterpart in the reference: EGG_SynC solves one
+ * array per call).  Many independent solves -- datasets of uneven size,
+ * eps sweeps over shared data -- in one call on device data. */
+typedef struct egg_sync_instance {
+    int64_t data_offset;     /* float offset of this instance's n x d rows in d_D */
+    int64_t n;
+    int32_t d;
+    float eps;
+} egg_sync_instance;
+
+typedef struct egg_sync_instance_result {
+    int32_t iterations;
+    int32_t status;          /* EGG_SYNC_OK or EGG_SYNC_CAPPED                    */
+    int32_t n_clusters;
+    int32_t n_cells;
+} egg_sync_instance_result;
+
+/* Solves inst[0..n_inst) with the options in *opt.  Labels are packed in
+ * instance order (instance k starts at row sum_{j<k} n_j of d_labels); final
+ * positions, if d_final is not NULL, likewise at float offset
+ * sum_{j<k} n_j d_j.  results[n_inst] and stats may be NULL.  Each instance's
+ * result equals what egg_sync_solve_device returns for it alone. */
+int egg_sync_solve_batch_device(const float *d_D, const egg_sync_instance *inst,
+                                int32_t n_inst, const egg_sync_options *opt,
+                                int32_t *d_labels, float *d_final,
+                                egg_sync_instance_result *results, egg_sync_stats *stats,
+                                void *stream);
 
 /* Grid build on device positions d_D (n x d) visited in order d_order_in
  * (point ids; NULL = identity).  Outputs (device, may be NULL):

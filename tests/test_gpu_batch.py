@@ -1,0 +1,128 @@
+"""GPU tests of the launch variants that must not change any result: the
+hipGraph-replayed loop, the tile-queue launch shapes, and the batched solve
+(many instances, several streams, optional graph) against single solves."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def S():
+    assert torch.cuda.is_available()
+    from egg_sync import solver
+    return solver
+
+
+def _same(a, b):
+    assert a["stats"]["iterations"] == b["stats"]["iterations"]
+    assert np.array_equal(a["labels"], b["labels"])
+    assert np.array_equal(a["X"].view(np.uint32), b["X"].view(np.uint32))
+
+
+def test_graph_replays_match_eager(S):
+    from egg_sync import data
+    X, eps = data.config1(seed=11, n=60000)
+    ref = S.solve(X, eps, iters_per_sync=2)
+    _say("eager:", ref["stats"]["iterations"], "iterations")
+    for rep in range(3):
+        g = S.solve(X, eps, iters_per_sync=2, use_graph=True)
+        _same(ref, g)
+        e = S.solve(X, eps, iters_per_sync=3)      # eager work between replays
+        _same(ref, e)
+    X2, _ = data.config1(seed=12, n=60000)         # same shape, new data: graph reused
+    _same(S.solve(X2, eps, iters_per_sync=2), S.solve(X2, eps, iters_per_sync=2, use_graph=True))
+
+
+def test_graph_timing_and_sync_mode(S):
+    from egg_sync import data, _native as nat
+    X, eps = data.config0(seed=0)
+    a = S.solve(X, eps, stop_rule=nat.STOP_RC)
+    b = S.solve(X, eps, stop_rule=nat.STOP_RC, use_graph=True, timing=True, iters_per_sync=5)
+    _same(a, b)
+    assert b["stats"]["stage_ms"][2] > 0
+
+
+@pytest.mark.parametrize("qt,qb", [(64, 0), (128, 0), (192, 0), (256, 7), (64, 1), (256, 300)])
+def test_queue_launch_shapes_are_bitwise_neutral(S, qt, qb):
+    from egg_sync import data
+    X, eps = data.config1(seed=13, n=40000)
+    ref = S.solve(X, eps)
+    _same(ref, S.solve(X, eps, queue_threads=qt, queue_blocks=qb))
+
+
+def _mixed_instances():
+    from egg_sync import data
+    rng = np.random.default_rng(5)
+    sets = []
+    for i, (n, d) in enumerate([(3000, 2), (20000, 3), (500, 5), (8000, 8), (1, 3), (12000, 3)]):
+        sets.append(data.gaussian_blobs(n, d, int(rng.integers(2, 8)), seed=100 + i,
+                                        center_lo=0.2, center_hi=0.8, sigma_lo=0.01,
+                                        sigma_hi=0.04, sizes="equal", noise=0.03))
+    eps = [[0.05, 0.02], [0.02], [0.1], [0.1, 0.2], [0.05], [0.01, 0.05]]
+    return sets, eps
+
+
+@pytest.mark.parametrize("streams,graph", [(1, False), (3, False), (4, True), (2, True)])
+def test_batch_equals_single_solves(S, streams, graph):
+    sets, eps = _mixed_instances()
+    D, specs = S.pack_instances(sets, eps)
+    specs.insert(3, (0, 0, 2, 0.05))               # an empty instance in the middle
+    r = S.solve_batch(D, specs, streams=streams, use_graph=graph, iters_per_sync=3)
+    _say(f"batch streams={streams} graph={graph}: {len(specs)} instances, "
+         f"{r['stats']['total_ms']:.1f} ms, max iterations {r['stats']['iterations']}")
+    k = 0
+    for X, es in zip(sets, eps):
+        for e in es:
+            if k == 3:
+                assert r["labels"][k].size == 0 and r["results"][k]["iterations"] == 0
+                k += 1
+            single = S.solve(X, e)
+            assert r["results"][k]["iterations"] == single["stats"]["iterations"], k
+            assert r["results"][k]["n_clusters"] == single["stats"]["n_clusters"], k
+            assert np.array_equal(r["labels"][k], single["labels"]), k
+            assert np.array_equal(r["X"][k].view(np.uint32), single["X"].view(np.uint32)), k
+            k += 1
+    assert k == len(specs)
+
+
+def test_batch_sync_mode_equals_single(S):
+    from egg_sync import data, _native as nat
+    sets = [data.config0(seed=s, n=n)[0] for s, n in ((0, 2000), (1, 700), (2, 1500))]
+    D, specs = S.pack_instances([(x, 0.05) for x in sets])
+    r = S.solve_batch(D, specs, stop_rule=nat.STOP_RC, streams=2)
+    for k, X in enumerate(sets):
+        single = S.solve(X, 0.05, stop_rule=nat.STOP_RC)
+        assert r["results"][k]["iterations"] == single["stats"]["iterations"]
+        assert np.array_equal(r["labels"][k], single["labels"])
+
+
+def test_batch_rejects_out_of_range_input(S):
+    from egg_sync import _native as nat
+    D = torch.rand(3000, device="cuda")
+    D[2500] = 1.5
+    with pytest.raises(nat.EggSyncError):
+        S.solve_batch(D, [(0, 1000, 2, 0.05), (2000, 500, 2, 0.05)])
+    r = S.solve_batch(D, [(0, 1000, 2, 0.05)])     # the bad value lies outside this instance
+    assert r["labels"][0].shape == (1000,)
+
+
+def test_batch_config3_sample_against_oracle(S, O):
+    from egg_sync import data
+    from egg_sync.metrics import same_partition
+    inst = data.config3_instances(seed=3, count=6, n_lo=1000, n_hi=6000)
+    D, specs = S.pack_instances(inst, [list(data.CONFIG3_EPS)] * len(inst))
+    r = S.solve_batch(D, specs, streams=4, use_graph=True)
+    k = 0
+    for X in inst:
+        for e in data.CONFIG3_EPS:
+            o = O.egg_solve(X, e)
+            assert r["results"][k]["iterations"] == o["iterations"], (k, X.shape, e)
+            assert same_partition(r["labels"][k], o["labels"]), (k, X.shape, e)
+            assert np.abs(r["X"][k] - o["X"]).max() <= 1e-5
+            k += 1
