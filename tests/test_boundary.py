@@ -110,3 +110,15 @@ def test_no_oracle_in_product_path():
                 assert not re.search(r"^\s*(import|from)\s+oracle\b", txt, re.M), f
                 assert "liboracle" not in txt and "egg_oracle" not in txt, f
                 assert "sync_ref" not in txt and "_ref/" not in txt, f
+
+
+def test_autotune_signatures_and_cache(tmp_path, monkeypatch):
+    from egg_sync import autotune
+    monkeypatch.setenv("EGG_SYNC_TUNE_CACHE", str(tmp_path / "t.json"))
+    monkeypatch.setattr(autotune, "_device_key", lambda: "dev")
+    assert autotune.plain_signature(1_000_000, 3, 0.02) == "plain|n2^20|d3|eps0.02"
+    sig = autotune.batch_signature([(0, 1000, 2, 0.1), (0, 1000, 2, 0.05), (2000, 4000, 8, 0.1)])
+    assert sig == "batch|k2^2|pts2^13|d2,8"
+    assert autotune.cached(sig) is None
+    autotune._store(sig, {"streams": 2}, [])
+    assert autotune.cached(sig) == {"streams": 2}

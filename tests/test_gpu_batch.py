@@ -126,3 +126,16 @@ def test_batch_config3_sample_against_oracle(S, O):
             assert same_partition(r["labels"][k], o["labels"]), (k, X.shape, e)
             assert np.abs(r["X"][k] - o["X"]).max() <= 1e-5
             k += 1
+
+
+def test_autotuner_checks_results_and_caches(S, tmp_path, monkeypatch):
+    from egg_sync import autotune
+    monkeypatch.setenv("EGG_SYNC_TUNE_CACHE", str(tmp_path / "tune.json"))
+    sets, eps = _mixed_instances()
+    D, specs = S.pack_instances(sets, eps)
+    space = {"streams": [2, 4], "use_graph": [False, True], "iters_per_sync": [4, 2]}
+    best, t, table = autotune.tune_batch(D, specs, repeats=1, space=space, log=_say)
+    assert len(table) >= 3 and t > 0
+    assert autotune.cached(autotune.batch_signature(specs)) == best
+    r = S.solve_batch(D, specs, **best)
+    assert r["results"][0]["iterations"] > 0
