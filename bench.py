@@ -11,7 +11,13 @@ the timed steps.  The CPU baseline is the reference's own CPU SynC
 (src/algorithms/SynC.cpp, compiled from source into oracle/_ref) running one
 OpenMP pass over a bounded sample of target points of the same data.
 
+The line also carries a "forms" object: the learned-jump form on n=4M,
+d=3, eps=0.01 (trained weights from egg_sync/weights, plain solve of the same
+data timed beside it, labels compared) and the batched form (64 datasets x 4
+eps values = 256 solves in one call).
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+                    [--forms plain,learned,batched]
 """
 import argparse
 import json
@@ -73,6 +79,66 @@ def cpu_baseline(X, eps, target_seconds=12.0):
                       f"{secs:.1f} s wall"}
 
 
+WEIGHTS = os.path.join(ROOT, "egg-sync_amd", "egg_sync", "weights", "jump_d3.npz")
+
+
+def learned_form(n=4_000_000):
+    """Learned-jump solve vs plain solve of the same data (n x 3, eps=0.01)."""
+    import torch
+    from egg_sync import data, learned, solver
+    from egg_sync.metrics import same_partition
+    if not os.path.exists(WEIGHTS):
+        return {"skipped": f"no trained weights at {os.path.relpath(WEIGHTS, ROOT)}"}
+    net = learned.JumpMLP.from_file(WEIGHTS)
+    X, eps = data.config4(n=n)
+    D = torch.from_numpy(X).cuda()
+    learned.solve_learned(D, eps, net)          # warm-up: workspace, code objects
+    solver.solve(D, eps, return_positions=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p = solver.solve(D, eps, return_positions=False)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    r = learned.solve_learned(D, eps, net)
+    same = bool(same_partition(r["labels"], p["labels"]))
+    return {"workload": f"n={n}, d=3, eps={eps}", "plain_s": t_plain,
+            "plain_iterations": p["stats"]["iterations"],
+            "plain_point_updates_per_s": n * p["stats"]["iterations"] / t_plain,
+            "learned_s": r["seconds"], "learned_passes": r["iterations"],
+            "speedup": t_plain / r["seconds"], "same_partition": same,
+            "result_used": "learned" if same else "plain (fallback)"}
+
+
+def batched_form(count=64):
+    """256 solves (64 datasets x 4 eps) in one batched call vs one call each."""
+    import torch
+    from egg_sync import data, solver
+    inst = data.config3_instances(seed=3, count=count)
+    D, specs = solver.pack_instances(inst, [list(data.CONFIG3_EPS)] * len(inst))
+    solver.solve_batch(D, specs, return_positions=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = solver.solve_batch(D, specs, return_positions=False)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    updates = sum(int(s[1]) * res["iterations"] for s, res in zip(specs, r["results"]))
+    t0 = time.perf_counter()
+    singles_same = True
+    for k, (o, nk, dk, e) in enumerate(specs):
+        x = D[o:o + nk * dk].view(nk, dk)
+        q = solver.solve(x, e, return_positions=False)
+        singles_same &= bool((q["labels"] == r["labels"][k]).all())
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    return {"workload": f"{len(specs)} solves: {count} datasets (n log-uniform in [1e3,2e5], "
+                        "d in {2,3,5,8}) x eps {0.01,0.02,0.05,0.1}",
+            "points": int(sum(int(s[1]) for s in specs)), "batch_s": t_batch,
+            "point_updates_per_s": updates / t_batch, "one_call_each_s": t_single,
+            "speedup_vs_one_call_each": t_single / t_batch,
+            "labels_equal_to_single_calls": singles_same,
+            "max_iterations": max(res["iterations"] for res in r["results"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +147,8 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--iters-per-sync", type=int, default=8)
+    ap.add_argument("--forms", default="plain,learned,batched",
+                    help="comma list; plain is always measured")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -163,6 +231,16 @@ def main():
                      "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
+    forms = {}
+    for name, fn in (("learned", learned_form), ("batched", batched_form)):
+        if name in args.forms.split(","):
+            try:
+                forms[name] = fn()
+            except Exception as e:  # report, keep the plain line
+                forms[name] = {"error": f"{type(e).__name__}: {e}"}
+            print(f"form {name}: {forms[name]}", file=sys.stderr, flush=True)
+    if forms:
+        line["forms"] = forms
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(X, eps)
     print(json.dumps(line), flush=True)

@@ -80,6 +80,12 @@ def config1(seed: int = 1, n: int = 1_000_000):
     return X, 0.02
 
 
+def config4(seed: int = 4, n: int = 4_000_000):
+    """Learned-jump evaluation workload: the config-1 generator at n=4,000,000, d=3, eps=0.01."""
+    X, _ = config1(seed=seed, n=n)
+    return X, 0.01
+
+
 def config2(seed: int = 2, n: int = 250_000):
     """Higher-dim sparse grid: n=250,000, d=8; 10 blobs sigma=0.03, centres U[0.2,0.8]^8,
     1% noise; eps=0.1."""

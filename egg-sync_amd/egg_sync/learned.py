@@ -115,6 +115,15 @@ class JumpMLP:
     def load(self, p):
         self.params.copy_(torch.from_numpy(np.asarray(p, np.float32)).cuda())
 
+    @classmethod
+    def from_file(cls, path: str) -> "JumpMLP":
+        """Network saved by `python -m egg_sync.train_jump` (a plain .npz, no pickles)."""
+        with np.load(path, allow_pickle=False) as z:
+            net = cls(int(z["n_out"]), hidden=int(z["hidden"]))
+            net.load(z["params"])
+            net.record = str(z["record"])
+        return net
+
 
 def features(X: torch.Tensor, prev: torch.Tensor, eps: float) -> torch.Tensor:
     n, d = X.shape
