@@ -139,6 +139,24 @@ def batched_form(count=64):
             "max_iterations": max(res["iterations"] for res in r["results"])}
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=5):
+    """Achievable HBM bandwidth: read + write bytes of a device-to-device copy / time."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (a.elapsed_time(b) / 1000.0) / 1e9
+    del src, dst
+    return gbs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +249,13 @@ def main():
                      "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
+    try:
+        cp = copy_bandwidth()
+        line["roofline"]["copy_gbs_measured"] = cp
+        line["roofline"]["frac_of_measured_copy"] = achieved / cp
+    except Exception as e:
+        line["roofline"]["copy_gbs_measured"] = None
+        print(f"copy bandwidth: {e}", file=sys.stderr)
     forms = {}
     for name, fn in (("learned", learned_form), ("batched", batched_form)):
         if name in args.forms.split(","):
