@@ -90,6 +90,7 @@ def learned_form(n=4_000_000):
     if not os.path.exists(WEIGHTS):
         return {"skipped": f"no trained weights at {os.path.relpath(WEIGHTS, ROOT)}"}
     net = learned.JumpMLP.from_file(WEIGHTS)
+    print("learned form: warm-up", file=sys.stderr, flush=True)
     X, eps = data.config4(n=n)
     D = torch.from_numpy(X).cuda()
     learned.solve_learned(D, eps, net)          # warm-up: workspace, code objects
@@ -115,13 +116,16 @@ def batched_form(count=64):
     from egg_sync import data, solver
     inst = data.config3_instances(seed=3, count=count)
     D, specs = solver.pack_instances(inst, [list(data.CONFIG3_EPS)] * len(inst))
+    print(f"batched form: {len(specs)} solves, warm-up", file=sys.stderr, flush=True)
     solver.solve_batch(D, specs, return_positions=False)
     torch.cuda.synchronize()
+    print("batched form: timed batch", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     r = solver.solve_batch(D, specs, return_positions=False)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     updates = sum(int(s[1]) * res["iterations"] for s, res in zip(specs, r["results"]))
+    print("batched form: one call per solve", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     singles_same = True
     for k, (o, nk, dk, e) in enumerate(specs):
@@ -267,6 +271,7 @@ def main():
     if forms:
         line["forms"] = forms
     if not args.no_cpu:
+        print("cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(X, eps)
     print(json.dumps(line), flush=True)
 

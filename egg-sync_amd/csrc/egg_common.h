@@ -123,6 +123,34 @@ __device__ __forceinline__ void box_dist(const int *g, const float *x, float cs,
     large = L;
 }
 
+// sin and cos of a small angle |v| <= 1 from their Taylor series about 0,
+// Horner form with fused multiply-adds: sin to the v^11 term, cos to v^12
+// (the first dropped terms, v^13/13! and v^14/14!, are below 2^-32 for
+// |v| <= 1).  Max error about 1 ulp against FP64 sin / cos on [-1, 1].  Beyond
+// |v| = 1 the library sinf / cosf are used.  The update kernel needs these
+// only for angles within a few cell sides of a cell corner.
+__host__ __device__ __forceinline__ void sincos_small(float v, float &s, float &c) {
+    if (!(fabsf(v) <= 1.0f)) {
+        s = sinf(v);
+        c = cosf(v);
+        return;
+    }
+    const float v2 = v * v;
+    float ps = -2.50521083854417e-08f;              // -1/11!
+    ps = fmaf(ps, v2, 2.75573192239859e-06f);       //  1/9!
+    ps = fmaf(ps, v2, -1.98412698412698e-04f);      // -1/7!
+    ps = fmaf(ps, v2, 8.33333333333333e-03f);       //  1/5!
+    ps = fmaf(ps, v2, -1.66666666666667e-01f);      // -1/3!
+    s = fmaf(v * v2, ps, v);
+    float pc = 2.08767569878681e-09f;               //  1/12!
+    pc = fmaf(pc, v2, -2.75573192239859e-07f);      // -1/10!
+    pc = fmaf(pc, v2, 2.48015873015873e-05f);       //  1/8!
+    pc = fmaf(pc, v2, -1.38888888888889e-03f);      // -1/6!
+    pc = fmaf(pc, v2, 4.16666666666667e-02f);       //  1/4!
+    pc = fmaf(pc, v2, -0.5f);                       // -1/2!
+    c = fmaf(v2, pc, 1.0f);
+}
+
 __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << __lane_id()) - 1ull;
 }

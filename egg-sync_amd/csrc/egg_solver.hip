@@ -791,16 +791,25 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
         load[j] += inst[k].n * inst[k].d;
     }
     const int K = std::max(1, opt.iters_per_sync);
+    // instances still iterating; finished ones are no longer enqueued
+    std::vector<char> live(n_inst, 0);
+    for (int k : active) live[k] = 1;
     auto enqueue_batch = [&](hipStream_t origin) -> int {
         int r = fork_streams(bw, origin, S);
         if (r) return r;
         for (int j = 0; j < S; j++)
             for (int k : per[j])
-                for (int it = 0; it < K; it++)
-                    enqueue_iteration(G[k], Ls[k], opt, bw.streams[j], nullptr);
+                if (live[k])
+                    for (int it = 0; it < K; it++)
+                        enqueue_iteration(G[k], Ls[k], opt, bw.streams[j], nullptr);
         return join_streams(bw, origin, S);
     };
-    if (opt.use_graph) {
+    // the graph holds the live set it was captured with; it is captured again
+    // when that set has shrunk to half, and plain launches cover the rest
+    int graph_live = 0;
+    auto ensure_graph = [&](int n_live) -> int {
+        if (!opt.use_graph) return 0;
+        if (bw.gexec && graph_live > 0 && 2 * n_live > graph_live) return 0;
         std::vector<char> key;
         key_add(key, bw.base);
         key_add(key, bw.states);
@@ -808,23 +817,30 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
         key_add(key, S);
         key_add(key, opt.stop_rule);
         for (int j = 0; j < S; j++)
-            for (int k : per[j]) {
-                key_add(key, k);
-                key_add(key, off[k]);
-                key_add(key, G[k]);
-            }
+            for (int k : per[j])
+                if (live[k]) {
+                    key_add(key, k);
+                    key_add(key, off[k]);
+                    key_add(key, G[k]);
+                }
         if (!bw.gexec || key != bw.gkey) {
             drop_graph(bw.gexec, bw.gkey);
             int inner = 0;
-            rc = capture_graph(bw.origin, bw.gexec, [&] { inner = enqueue_batch(bw.origin); });
-            if (rc) return rc;
+            const int r = capture_graph(bw.origin, bw.gexec,
+                                        [&] { inner = enqueue_batch(bw.origin); });
+            if (r) return r;
             if (inner) {
                 drop_graph(bw.gexec, bw.gkey);
                 return inner;
             }
             bw.gkey = key;
         }
-    }
+        graph_live = n_live;
+        return 0;
+    };
+    int n_live = (int)active.size();
+    rc = ensure_graph(n_live);
+    if (rc) return rc;
     Events E;
     if (opt.timing) {
         rc = E.ensure(4);
@@ -834,7 +850,7 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
     const auto t_setup = clk::now();
     int max_iter_seen = 0;
     for (;;) {
-        if (opt.use_graph) {
+        if (opt.use_graph && graph_live == n_live) {
             HIP_TRY(hipGraphLaunch(bw.gexec, s));
         } else {
             rc = enqueue_batch(s);
@@ -844,12 +860,17 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
         HIP_TRY(hipMemcpyAsync(bw.h_states, bw.states, sizeof(DevState) * n_inst,
                                hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        bool all = true;
+        n_live = 0;
         for (int k : active) {
-            all = all && bw.h_states[k].done;
+            live[k] = !bw.h_states[k].done;
+            n_live += live[k];
             max_iter_seen = std::max(max_iter_seen, bw.h_states[k].iter);
         }
-        if (all) break;
+        if (n_live == 0) break;
+        if (opt.use_graph && n_live != graph_live) {
+            rc = ensure_graph(n_live);
+            if (rc) return rc;
+        }
     }
     if (opt.timing) HIP_TRY(hipEventRecord(E.ev[1], s));
     // final grids and labels (EGG_SynC.cu:1504-1541), one per instance

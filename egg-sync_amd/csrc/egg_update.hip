@@ -245,8 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
             sx[l] = sinf(x[l]);
             cx[l] = cosf(x[l]);
             const float u = x[l] - a0[l];
-            su[l] = sinf(u);
-            cu[l] = cosf(u);
+            sincos_small(u, su[l], cu[l]);
             border |= active && on_border(x[l], g.cs);
         }
         const int g0od = select_dim<D>(g0, od);
@@ -284,24 +283,47 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                 const int r0 = cell_run[c], r1 = cell_run[c + 1];
                 for (int base = r0; base < r1; base += kWave) {
                     const int rr = base + lane;
+                    int my_len = 1;
                     if (rr < r1) {
                         const int j = run_start[rr];
-                        len[lane] = run_start[rr + 1] - j;
+                        my_len = run_start[rr + 1] - j;
+                        len[lane] = my_len;
 #pragma unroll
                         for (int l = 0; l < D; l++) {
                             const float xq = Xs[(long long)l * n + j];
                             src[lane].x[l] = xq;
                             if (any_part) {
                                 const float v = xq - a0[l];
-                                src[lane].sv[l] = sinf(v);
-                                src[lane].cv[l] = cosf(v);
+                                sincos_small(v, src[lane].sv[l], src[lane].cv[l]);
                             }
                         }
                     }
+                    const bool multi = __any(my_len > 1);
                     wave_sync();
                     const int m = min(kWave, r1 - base);
-                    if (part) {
+                    if (part && !multi) {
+                        // every run of the chunk is a single point
 #pragma unroll 4
+                        for (int k = 0; k < m; k++) {
+                            const SrcPt<D> &q = src[k];
+                            float s = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float df = x[l] - q.x[l];
+                                s += df * df;
+                            }
+                            if (s <= thr) {
+#pragma unroll
+                                for (int l = 0; l < D; l++) {
+                                    const float p1 = q.sv[l] * cu[l];
+                                    const float p2 = q.cv[l] * su[l];
+                                    sml[l] += p1 - p2;
+                                }
+                                N += 1;
+                            }
+                        }
+                    } else if (part) {
+#pragma unroll 2
                         for (int k = 0; k < m; k++) {
                             const SrcPt<D> &q = src[k];
                             float s = 0.f;

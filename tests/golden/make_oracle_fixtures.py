@@ -1,9 +1,17 @@
-"""Writes tests/golden/c1_full_oracle.npz: the CPU oracle's EGG-SynC solve of the
-full config-1 workload (n=1,000,000, d=3, eps=0.02; about 7 minutes on 8 cores).
+"""Writes the full-size oracle fixtures, so GPU runs at full size are compared
+with the CPU oracle without re-running it:
 
-Stored: labels (int32, point-id order), iteration count, a seeded sample of
-20,000 final positions and the column sums of all final positions, so a GPU
-run at full size can be compared with the oracle without re-running it.
+  c1_full_oracle.npz  config 1 (n=1,000,000, d=3, eps=0.02; ~7 min on 8 cores):
+                      labels (int32, point-id order), iteration count, a seeded
+                      sample of 20,000 final positions, column sums of all final
+                      positions.
+  c2_full_oracle.npz  config 2 (n=250,000, d=8, eps=0.1; ~3 min): the same from the
+                      FP32 oracle, plus the FP64 oracle's iteration count, labels
+                      and sampled final positions, and the order parameter r_c of
+                      the FP64 final positions computed entirely in FP64
+                      (brute force over distinct positions with multiplicities).
+
+    python tests/golden/make_oracle_fixtures.py [c1] [c2]
 """
 import os
 import sys
@@ -18,7 +26,20 @@ import oracle as O  # noqa: E402
 from egg_sync import data  # noqa: E402
 
 
-def main():
+def rc_fp64(X, eps):
+    """r = (1/n) sum_p (1/|N_p|) sum_{j in N_p} exp(-|x_j - x_p|), N_p = {j: |x_j - x_p| <= eps}
+    (SynC.cpp:42-56), all in FP64, over distinct positions weighted by multiplicity."""
+    U, inv, cnt = np.unique(np.asarray(X, np.float64), axis=0, return_inverse=True,
+                            return_counts=True)
+    r_u = np.empty(len(U))
+    for a in range(0, len(U), 256):
+        dist = np.sqrt(((U[a:a + 256, None, :] - U[None, :, :]) ** 2).sum(-1))
+        w = (dist <= eps) * cnt[None, :]
+        r_u[a:a + 256] = (w * np.exp(-dist)).sum(1) / w.sum(1)
+    return float((r_u[inv.ravel()]).mean())
+
+
+def c1():
     X, eps = data.config1()
     r = O.egg_solve(X, eps)
     rng = np.random.default_rng(0)
@@ -26,8 +47,28 @@ def main():
     np.savez_compressed(os.path.join(HERE, "c1_full_oracle.npz"), labels=r["labels"],
                         iterations=np.int32(r["iterations"]), sample_idx=idx,
                         sample_X=r["X"][idx], X_sum=r["X"].astype(np.float64).sum(0))
-    print("iterations", r["iterations"], "clusters", int(r["labels"].max()) + 1)
+    print("c1 iterations", r["iterations"], "clusters", int(r["labels"].max()) + 1, flush=True)
+
+
+def c2():
+    X, eps = data.config2()
+    r = O.egg_solve(X, eps)
+    print("c2 fp32 iterations", r["iterations"], flush=True)
+    r64 = O.egg_solve(X, eps, fp64=True)
+    print("c2 fp64 iterations", r64["iterations"], flush=True)
+    rng = np.random.default_rng(0)
+    idx = np.sort(rng.choice(len(X), 5000, replace=False)).astype(np.int32)
+    rc64 = rc_fp64(r64["X"], eps)
+    np.savez_compressed(os.path.join(HERE, "c2_full_oracle.npz"), labels=r["labels"],
+                        iterations=np.int32(r["iterations"]), sample_idx=idx,
+                        sample_X=r["X"][idx], X_sum=r["X"].astype(np.float64).sum(0),
+                        labels64=r64["labels"], iterations64=np.int32(r64["iterations"]),
+                        sample_X64=r64["X"][idx], rc64=np.float64(rc64))
+    print("c2 clusters", int(r["labels"].max()) + 1, "rc64", rc64, flush=True)
 
 
 if __name__ == "__main__":
-    main()
+    O.build()
+    todo = sys.argv[1:] or ["c1", "c2"]
+    for name in todo:
+        {"c1": c1, "c2": c2}[name]()

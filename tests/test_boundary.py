@@ -122,3 +122,44 @@ def test_autotune_signatures_and_cache(tmp_path, monkeypatch):
     assert autotune.cached(sig) is None
     autotune._store(sig, {"streams": 2}, [])
     assert autotune.cached(sig) == {"streams": 2}
+
+
+def test_small_angle_sincos_error_bound(tmp_path):
+    """sincos_small (egg_common.h), compiled for the host by hipcc, against FP64
+    sin / cos: at most 1.5 ulp on [-1, 1] (and the library functions beyond)."""
+    import shutil
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "sc.cpp"
+    src.write_text(r'''
+#include <cmath>
+#include <cstdio>
+#include "%s"
+int main() {
+    double worst_s = 0, worst_c = 0;
+    for (int i = -2000000; i <= 2000000; i++) {
+        const float v = i * 5e-7f;
+        float s, c;
+        egg::sincos_small(v, s, c);
+        const double es = std::fabs((double)s - std::sin((double)v)) /
+                          std::ldexp(1.0, std::ilogb(std::fabs(std::sin((double)v)) + 1e-300) - 23);
+        const double ec = std::fabs((double)c - std::cos((double)v)) /
+                          std::ldexp(1.0, std::ilogb(std::cos((double)v)) - 23);
+        if (v != 0.0f && es > worst_s) worst_s = es;
+        if (ec > worst_c) worst_c = ec;
+    }
+    float s, c;
+    egg::sincos_small(3.0f, s, c);
+    std::printf("%%.3f %%.3f %%d\n", worst_s, worst_c,
+                (int)(s == std::sin(3.0f) && c == std::cos(3.0f)));
+    return 0;
+}
+''' % os.path.join(ROOT, "egg-sync_amd", "csrc", "egg_common.h"))
+    exe = tmp_path / "sc"
+    subprocess.run([hipcc, "-O2", "-ffp-contract=off", "-std=c++17", str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    ws, wc, big = subprocess.run([str(exe)], capture_output=True, text=True,
+                                 check=True).stdout.split()
+    assert float(ws) <= 1.5 and float(wc) <= 1.5, (ws, wc)
+    assert big == "1"

@@ -250,3 +250,27 @@ def test_config1_sample_solve_matches_oracle(S, O):
     assert a["stats"]["iterations"] == b["iterations"]
     assert O.same_partition(a["labels"], b["labels"])
     assert np.abs(a["X"] - b["X"]).max() <= 1e-5
+
+
+def test_full_size_config2_against_fp32_and_fp64_oracle(S, O):
+    """n=250,000, d=8, eps=0.1 (od=3: 175,616 outer cells) against the
+    committed oracle runs: FP32 exact-mimic (labels, iterations, positions)
+    and FP64 (positions, and r_c computed entirely in FP64)."""
+    from egg_sync import data
+    g = golden("c2_full_oracle")
+    X, eps = data.config2()
+    _say("config2 full", X.shape, eps)
+    a = S.solve(X, eps, compute_rc=True)
+    _say("config2 iterations", a["stats"]["iterations"], "rc", a["stats"]["rc"])
+    assert a["stats"]["iterations"] == int(g["iterations"])
+    assert O.same_partition(a["labels"], g["labels"])
+    assert O.same_partition(a["labels"], g["labels64"])
+    idx = g["sample_idx"]
+    err32 = float(np.abs(a["X"][idx] - g["sample_X"]).max())
+    err64 = float(np.abs(a["X"][idx].astype(np.float64) - g["sample_X64"]).max())
+    drc = abs(a["stats"]["rc"] - float(g["rc64"]))
+    _say(f"config2: max |x - x_oracle32| {err32:.3g}, max |x - x_oracle64| {err64:.3g}, "
+         f"|rc - rc_oracle64| {drc:.3g}")
+    assert err32 <= 1e-5 and err64 <= 1e-5
+    assert np.abs(a["X"].astype(np.float64).sum(0) - g["X_sum"]).max() <= 1e-5 * len(X)
+    assert drc <= 1e-6
