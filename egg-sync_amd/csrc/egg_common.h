@@ -26,7 +26,9 @@ constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kSortItems = 8;     // items per thread in radix passes
 constexpr int kSortTile = kBlock * kSortItems;
 constexpr int kSumChunk = 2048;   // max points per summary work item
-constexpr int kTile = 64;         // max points per update tile (one wavefront)
+constexpr int kTile = 64;         // max runs per update tile (one wavefront)
+constexpr int kDefaultGroup = 1;  // default cells per tile group along the last coordinate
+constexpr int kAnchorBlock = 8;   // group widths are powers of two dividing this
 
 // Grid geometry and thresholds, computed on the host (EGG_SynC.cu:1249-1260).
 struct Geo {
@@ -51,6 +53,7 @@ struct Geo {
     int wpow[kMaxD];   // w^l, l < od
     int qblocks;       // workgroups of the tile-queue kernels (0 = automatic)
     int qthreads;      // threads per workgroup of those kernels (64..256, multiple of 64)
+    int gw;            // update-tile groups: cells per block of the last coordinate (<= 1: one cell)
 };
 
 // Device-resident loop state (one per solve).
@@ -71,6 +74,7 @@ struct DevState {
     int tile_next;     // work-queue head of the update kernel
     int xtile_next;    // work-queue head of the extra check
     int runs;          // runs of equal positions in the sorted order
+    int G;             // cell groups of the update tiles
     double rc;         // order parameter of the last r_c evaluation
 };
 

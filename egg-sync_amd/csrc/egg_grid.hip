@@ -261,15 +261,58 @@ __global__ __launch_bounds__(kBlock) void k_runs(const int *__restrict__ head,
     }
 }
 
-// Update tiles per cell: one per 64 runs.
+// Group heads of the update tiles.  Cells are sorted by (outer id, g_od, ...,
+// g_{d-1}), so cells that share the outer cell and every inner coordinate but
+// the last are adjacent; such a cell starts a new group when its last
+// coordinate leaves the previous cell's block of gw cells.  With no inner
+// dimension (od == d) or gw <= 1 every cell is its own group.
+__global__ __launch_bounds__(kBlock) void k_group_heads(const int *__restrict__ cell_g,
+                                                        const int *__restrict__ cell_outer,
+                                                        Geo g, const DevState *st,
+                                                        int *__restrict__ gflag,
+                                                        DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int C = st->C, n = g.n, d = g.d, last = g.d - 1;
+    const bool grouped = g.gw > 1 && g.od < g.d;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+        bool h = c == 0 || !grouped;
+        if (!h) {
+            h = cell_outer[c] != cell_outer[c - 1];
+            for (int l = g.od; l < last && !h; l++)
+                h = cell_g[(long long)l * n + c] != cell_g[(long long)l * n + c - 1];
+            if (!h)
+                h = cell_g[(long long)last * n + c] / g.gw !=
+                    cell_g[(long long)last * n + c - 1] / g.gw;
+        }
+        gflag[c] = h ? 1 : 0;
+        (void)d;
+    }
+}
+
+// First cell of every group; grp_cell[G] = C.
+__global__ __launch_bounds__(kBlock) void k_group_cells(const int *__restrict__ gflag,
+                                                        const int *__restrict__ gpos,
+                                                        const DevState *st,
+                                                        int *__restrict__ grp_cell,
+                                                        DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int C = st->C;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+        if (gflag[c]) grp_cell[gpos[c]] = c;
+        if (c == C - 1) grp_cell[gpos[c] + gflag[c]] = C;
+    }
+}
+
+// Update tiles per group: one per 64 runs.
 __global__ __launch_bounds__(kBlock) void k_tile_count(const int *__restrict__ cell_run,
+                                                       const int *__restrict__ grp_cell,
                                                        const DevState *st,
                                                        int *__restrict__ ntile,
                                                        DevState *st_loop) {
     if (st_loop && st_loop->done) return;
-    const int C = st->C;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x)
-        ntile[c] = (cell_run[c + 1] - cell_run[c] + kTile - 1) / kTile;
+    const int G = st->G;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x)
+        ntile[k] = (cell_run[grp_cell[k + 1]] - cell_run[grp_cell[k]] + kTile - 1) / kTile;
 }
 
 // One wavefront per work item (<= kSumChunk points of one cell).  Per dim:
@@ -380,12 +423,13 @@ __global__ __launch_bounds__(kBlock) void k_summary_combine(
     }
 }
 
-// Total of a per-cell exclusive scan (summary work items or update tiles).
+// Total of an exclusive scan of *count values (summary work items per cell,
+// update tiles per group).
 __global__ void k_store_total(const int *__restrict__ off, const int *__restrict__ cnt,
-                              DevState *st, int *out, DevState *st_loop) {
+                              const int *count, int *out, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
-    const int C = st->C;
-    *out = C > 0 ? off[C - 1] + cnt[C - 1] : 0;
+    const int m = *count;
+    *out = m > 0 ? off[m - 1] + cnt[m - 1] : 0;
 }
 
 void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
@@ -393,7 +437,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     const int gb = grid_for(n);
     k_wi_count<<<gb, kBlock, 0, s>>>(b.cell_start, st, b.nchunk, st_loop);
     scan_exclusive(b.nchunk, b.wi_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
-    k_store_total<<<1, 1, 0, s>>>(b.wi_off, b.nchunk, st, &st->W, st_loop);
+    k_store_total<<<1, 1, 0, s>>>(b.wi_off, b.nchunk, &st->C, &st->W, st_loop);
     // work items <= C + n / kSumChunk <= n + n / kSumChunk
     const long long max_wi = (long long)n + n / kSumChunk + 1;
     const int wb = grid_for(max_wi * kWave, kBlock, 16384);
@@ -421,9 +465,12 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     scan_exclusive(b.head, b.hpos, n, nullptr, b.scan_tmp, nullptr, st_loop, s);
     k_runs<<<gb, kBlock, 0, s>>>(b.head, b.hpos, b.point_cell, b.cell_start, n, b.run_start,
                                  b.cell_run, st, st_loop);
-    k_tile_count<<<gb, kBlock, 0, s>>>(b.cell_run, st, b.ntile, st_loop);
-    scan_exclusive(b.ntile, b.tile_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
-    k_store_total<<<1, 1, 0, s>>>(b.tile_off, b.ntile, st, &st->T, st_loop);
+    k_group_heads<<<gb, kBlock, 0, s>>>(b.cell_g, b.cell_outer, g, st, b.gflag, st_loop);
+    scan_exclusive(b.gflag, b.gpos, n, &st->C, b.scan_tmp, &st->G, st_loop, s);
+    k_group_cells<<<gb, kBlock, 0, s>>>(b.gflag, b.gpos, st, b.grp_cell, st_loop);
+    k_tile_count<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.ntile, st_loop);
+    scan_exclusive(b.ntile, b.tile_off, n, &st->G, b.scan_tmp, nullptr, st_loop, s);
+    k_store_total<<<1, 1, 0, s>>>(b.tile_off, b.ntile, &st->G, &st->T, st_loop);
 }
 
 }  // namespace egg

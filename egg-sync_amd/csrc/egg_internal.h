@@ -41,7 +41,12 @@ struct GridBufs {
     // order); head / hpos are reused for the run-head flags and their scan
     int *run_start;      // [n+1] first sorted position of each run
     int *cell_run;       // [n+1] first run of each cell
-    int *ntile, *tile_off;// [n]  update tiles per cell (<= 64 runs each), exclusive scan
+    // update tiles: cells are grouped (consecutive cells of one outer cell,
+    // equal inner coordinates but the last, last one in the same block of gw
+    // cells); a tile holds <= 64 runs of one group
+    int *gflag, *gpos;   // [n] group-head flag of each cell and its exclusive scan
+    int *grp_cell;       // [n+1] first cell of each group, grp_cell[G] = C
+    int *ntile, *tile_off;// [n]  tiles per group, exclusive scan
     float *rres;         // [d][n] new position of each run
     int *rdbg;           // [4n]  N / half / border / synced of each run (diagnostics)
     float *Ts, *Tc;      // [d][n]

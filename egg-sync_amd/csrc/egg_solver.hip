@@ -170,6 +170,9 @@ Layout carve(void *base, const Geo &g) {
     b.wi_off = c.take<int>(n);
     b.run_start = c.take<int>(n + 1);
     b.cell_run = c.take<int>(n + 1);
+    b.gflag = c.take<int>(n);
+    b.gpos = c.take<int>(n);
+    b.grp_cell = c.take<int>(n + 1);
     b.ntile = c.take<int>(n);
     b.tile_off = c.take<int>(n);
     b.rres = c.take<float>(n * d);
@@ -319,11 +322,16 @@ void default_options(egg_sync_options &o) {
     o.queue_blocks = 0;
     o.queue_threads = 0;
     o.streams = 0;
+    o.tile_group = 0;
 }
 
 void apply_tuning(const egg_sync_options &opt, Geo &g) {
     g.qblocks = std::max(0, opt.queue_blocks);
     g.qthreads = opt.queue_threads;
+    int gw = opt.tile_group > 0 ? opt.tile_group : kDefaultGroup;
+    gw = std::min(gw, kAnchorBlock);
+    while (gw & (gw - 1)) gw &= gw - 1;  // a power of two: groups never straddle an anchor block
+    g.gw = gw;
 }
 
 template <typename T>

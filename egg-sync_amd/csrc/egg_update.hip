@@ -180,15 +180,59 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Cell of tile t: the largest c with tile_off[c] <= t.
-__device__ __forceinline__ int tile_cell(const int *__restrict__ tile_off, int C, int t) {
-    int lo = 0, hi = C - 1;
+// Group of tile t: the largest k with tile_off[k] <= t.
+__device__ __forceinline__ int tile_group(const int *__restrict__ tile_off, int G, int t) {
+    int lo = 0, hi = G - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (tile_off[mid] <= t) lo = mid;
         else hi = mid - 1;
     }
     return lo;
+}
+
+// A tile seen by one lane: the tile's group (cells [gc0, gc1), all in one
+// outer cell, equal in every inner coordinate but the last), the lane's run
+// r (active if it belongs to the tile), its first sorted position i and its
+// cell cl.
+struct TileLane {
+    int gc0, gc1, r, i, cl;
+    bool active;
+};
+
+__device__ __forceinline__ TileLane tile_lane(int t, const int *__restrict__ tile_off, int G,
+                                              const int *__restrict__ grp_cell,
+                                              const int *__restrict__ cell_run,
+                                              const int *__restrict__ run_start,
+                                              const int *__restrict__ point_cell) {
+    TileLane L;
+    const int k = tile_group(tile_off, G, t);
+    L.gc0 = grp_cell[k];
+    L.gc1 = grp_cell[k + 1];
+    L.r = cell_run[L.gc0] + (t - tile_off[k]) * kTile + __lane_id();
+    L.active = L.r < cell_run[L.gc1];
+    L.i = L.active ? run_start[L.r] : 0;
+    L.cl = L.active ? point_cell[L.i] : L.gc0;
+    return L;
+}
+
+// cell_in_window for a group: the window of the last dimension spans
+// [g0[d-1] - rad, ghi + rad] (ghi = last coordinate of the group's last cell).
+template <int D>
+__device__ __forceinline__ bool cell_in_group_window(const int *__restrict__ cell_g, int n, int c,
+                                                     const int *g0, int od, int rad, int ghi,
+                                                     int *gc, bool *stop) {
+#pragma unroll
+    for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + c];
+    const int hi_od = od == D - 1 ? ghi : select_dim<D>(g0, od);
+    *stop = od < D && select_dim<D>(gc, od) > hi_od + rad;
+    bool in = true;
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        const int hi = l == D - 1 ? ghi : g0[l];
+        if (l > od && (gc[l] - hi > rad || g0[l] - gc[l] > rad)) in = false;
+    }
+    return in && !*stop;
 }
 
 __device__ __forceinline__ int next_tile(int *counter) {
@@ -211,13 +255,14 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
     const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ run_start,
-    const int *__restrict__ cell_run, const int *__restrict__ tile_off,
+    const int *__restrict__ cell_run, const int *__restrict__ grp_cell,
+    const int *__restrict__ point_cell, const int *__restrict__ tile_off,
     float *__restrict__ rres, int *__restrict__ rdbg, Geo g, DevState *st) {
     if (st->done) return;
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
     __shared__ int s_len[kBlock / kWave][kWave];
     const int n = g.n, R = g.R, od = g.od;
-    const int C = st->C, T = st->T;
+    const int G = st->G, T = st->T;
     const float thr = g.thr_eps, thr_half = g.thr_half;
     const int lane = __lane_id();
     SrcPt<D> *src = s_src[threadIdx.x / kWave];
@@ -227,18 +272,21 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     for (;;) {
         const int t = next_tile(&st->tile_next);
         if (t >= T) break;
-        const int c0 = tile_cell(tile_off, C, t);
-        const int own = cell_start[c0 + 1] - cell_start[c0];
-        const int r = cell_run[c0] + (t - tile_off[c0]) * kTile + lane;
-        const bool active = r < cell_run[c0 + 1];
-        const int i = active ? run_start[r] : 0;
+        const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
+        const int c0 = tl.gc0, cl = tl.cl, r = tl.r, i = tl.i;
+        const bool active = tl.active;
+        const int own = cell_start[cl + 1] - cell_start[cl];
+        const int ghi = cell_g[(long long)(D - 1) * n + tl.gc1 - 1];
         float x[D], a0[D], sx[D], cx[D], su[D], cu[D], sum[D], sml[D];
         int g0[D];
         bool border = false;
 #pragma unroll
         for (int l = 0; l < D; l++) {
             g0[l] = cell_g[(long long)l * n + c0];
-            a0[l] = (float)g0[l] * g.cs;
+            // anchor: the corner of the tile's cell in the outer dimensions
+            // and of its block of kAnchorBlock cells in the inner ones, the
+            // same for every group width that divides kAnchorBlock
+            a0[l] = (float)(l < od ? g0[l] : g0[l] & ~(kAnchorBlock - 1)) * g.cs;
             x[l] = active ? Xs[(long long)l * n + i] : a0[l];
             sum[l] = 0.f;
             sml[l] = 0.f;
@@ -255,7 +303,8 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
             for (int c = cb; c < ce; c++) {
                 int gc[D];
                 bool stop;
-                const bool in = cell_in_window<D>(cell_g, n, c, g0, od, R + 1, g0od, gc, &stop);
+                const bool in =
+                    cell_in_group_window<D>(cell_g, n, c, g0, od, R + 1, ghi, gc, &stop);
                 if (stop) break;
                 if (!in) continue;
                 float sm, lg;
@@ -272,9 +321,9 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                     }
                     const int cnt = cell_start[c + 1] - cell_start[c];
                     N += cnt;
-                    if (border && c == c0) half += cnt;
+                    if (border && c == cl) half += cnt;
                 }
-                const bool need_half = full && border && c != c0;
+                const bool need_half = full && border && c != cl;
                 const bool any_part = __any(part);
                 if (!any_part && !__any(need_half)) continue;
                 // the cell's runs, 64 per chunk: one test per run, each
@@ -479,13 +528,14 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_extra_tile(
     const float *__restrict__ Xs, const int *__restrict__ cell_g, const int *__restrict__ ob,
     const int *__restrict__ oe, const int *__restrict__ run_start,
-    const int *__restrict__ cell_run, const int *__restrict__ tile_off, int *__restrict__ pairs,
-    int pair_cap, Geo g, DevState *st) {
+    const int *__restrict__ cell_run, const int *__restrict__ grp_cell,
+    const int *__restrict__ point_cell, const int *__restrict__ tile_off,
+    int *__restrict__ pairs, int pair_cap, Geo g, DevState *st) {
     if (st->done || st->flag == 0) return;
     __shared__ float s_x[kBlock / kWave][kWave][SrcPt<D>::P];
     __shared__ int s_pos[kBlock / kWave][kWave];
     const int n = g.n, od = g.od, R = g.R_extra;
-    const int C = st->C, T = st->T;
+    const int G = st->G, T = st->T;
     const int lane = __lane_id();
     float(*src)[SrcPt<D>::P] = s_x[threadIdx.x / kWave];
     int *src_pos = s_pos[threadIdx.x / kWave];
@@ -495,10 +545,10 @@ __global__ __launch_bounds__(kBlock) void k_extra_tile(
         const int t = next_tile(&st->xtile_next);
         if (t >= T) break;
         if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-        const int c0 = tile_cell(tile_off, C, t);
-        const int r = cell_run[c0] + (t - tile_off[c0]) * kTile + lane;
-        const bool active = r < cell_run[c0 + 1];
-        const int i = active ? run_start[r] : 0;
+        const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
+        const int c0 = tl.gc0, i = tl.i;
+        const bool active = tl.active;
+        const int ghi = cell_g[(long long)(D - 1) * n + tl.gc1 - 1];
         float xp[D];
         int g0[D];
 #pragma unroll
@@ -530,7 +580,8 @@ __global__ __launch_bounds__(kBlock) void k_extra_tile(
             for (int c = cb; c < ce; c++) {
                 int gc[D];
                 bool stop;
-                const bool in = cell_in_window<D>(cell_g, n, c, g0, od, R + 1, g0od, gc, &stop);
+                const bool in =
+                    cell_in_group_window<D>(cell_g, n, c, g0, od, R + 1, ghi, gc, &stop);
                 if (stop) break;
                 if (!in) continue;
                 float sm, lg;
@@ -795,8 +846,9 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
 #define UPD_CASE(DD)                                                                          \
     case DD:                                                                                  \
         k_update_tile<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc,     \
-                                                b.ob, b.oe, b.run_start, b.cell_run,          \
-                                                b.tile_off, b.rres, rdbg, g, st);             \
+                                            b.ob, b.oe, b.run_start, b.cell_run,              \
+                                            b.grp_cell, b.point_cell, b.tile_off, b.rres,     \
+                                            rdbg, g, st);                                     \
         k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.head, b.hpos, b.rres, rdbg,     \
                                                             b.perm_s, b.X, b.perm, g.n, st,   \
                                                             d);                               \
@@ -813,8 +865,8 @@ void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s) 
 #define EX_CASE(DD)                                                                           \
     case DD:                                                                                  \
         k_extra_tile<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_g, b.ob, b.oe, b.run_start,       \
-                                               b.cell_run, b.tile_off, b.pairs, b.pair_cap, g, \
-                                               st);                                           \
+                                           b.cell_run, b.grp_cell, b.point_cell, b.tile_off,  \
+                                           b.pairs, b.pair_cap, g, st);                       \
         k_extra_mbr<DD><<<1024, kBlock, 0, s>>>(b.Xs, b.cell_g, b.ob, b.oe, b.run_start,      \
                                                 b.cell_run, b.pairs, b.pair_cap, g, st);      \
         break;

@@ -26,12 +26,13 @@ def _stream():
 
 def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-3, max_iter=0,
           compute_rc=False, timing=False, iters_per_sync=4, return_positions=True,
-          itr_cap=16384, use_graph=False, queue_blocks=0, queue_threads=0):
+          itr_cap=16384, use_graph=False, queue_blocks=0, queue_threads=0, tile_group=0):
     """Solve on the GPU.  Returns dict(labels int32 numpy, X final numpy or None, stats).
 
     use_graph replays the loop body from a captured hipGraph; queue_blocks /
-    queue_threads set the launch shape of the tile-queue kernels (0 = automatic).
-    None of these change the arithmetic."""
+    queue_threads set the launch shape of the tile-queue kernels (0 = automatic);
+    tile_group lets up to that many neighbouring cells share an update tile.
+    None of these change the result."""
     D = _dev_f32(X)
     n, d = D.shape
     if label_rule is None:
@@ -39,7 +40,8 @@ def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-
     opt = nat.default_options(stop_rule=stop_rule, label_rule=label_rule, lam=lam,
                               max_iter=max_iter, compute_rc=int(compute_rc), timing=int(timing),
                               iters_per_sync=iters_per_sync, use_graph=int(use_graph),
-                              queue_blocks=queue_blocks, queue_threads=queue_threads)
+                              queue_blocks=queue_blocks, queue_threads=queue_threads,
+                              tile_group=tile_group)
     labels = torch.empty(n, dtype=torch.int32, device=D.device)
     final = torch.empty_like(D) if return_positions else None
     st = nat.Stats()
@@ -91,7 +93,8 @@ def pack_instances(datasets, eps_values=None):
 
 def solve_batch(D: torch.Tensor, specs, *, stop_rule=nat.STOP_EGG, label_rule=None,
                 lam=1 - 1e-3, max_iter=0, timing=False, iters_per_sync=4, use_graph=False,
-                streams=4, queue_blocks=0, queue_threads=0, return_positions=True):
+                streams=4, queue_blocks=0, queue_threads=0, tile_group=0,
+                return_positions=True):
     """Many independent solves in one call (egg_sync_solve_batch_device).
 
     `D` is a flat float32 device tensor, `specs` a list of (data_offset, n, d,
@@ -103,7 +106,7 @@ def solve_batch(D: torch.Tensor, specs, *, stop_rule=nat.STOP_EGG, label_rule=No
                               max_iter=max_iter, timing=int(timing),
                               iters_per_sync=iters_per_sync, use_graph=int(use_graph),
                               queue_blocks=queue_blocks, queue_threads=queue_threads,
-                              streams=streams)
+                              streams=streams, tile_group=tile_group)
     k = len(specs)
     inst = (nat.Instance * max(k, 1))()
     rows = fl = 0

@@ -56,6 +56,17 @@ def test_queue_launch_shapes_are_bitwise_neutral(S, qt, qb):
     _same(ref, S.solve(X, eps, queue_threads=qt, queue_blocks=qb))
 
 
+@pytest.mark.parametrize("gw", [2, 4, 8, 3])
+def test_tile_groups_are_bitwise_neutral(S, gw):
+    from egg_sync import data
+    cases = [(*data.config1(seed=14, n=40000),),
+             (data.gaussian_blobs(6000, 5, 4, seed=15, center_lo=0.2, center_hi=0.8,
+                                  sigma=0.03, noise=0.05), 0.1),
+             (data.config2(n=5000)[0], 0.1)]
+    for X, eps in cases:
+        _same(S.solve(X, eps, tile_group=1), S.solve(X, eps, tile_group=gw))
+
+
 def _mixed_instances():
     from egg_sync import data
     rng = np.random.default_rng(5)
