@@ -155,6 +155,30 @@ __host__ __device__ __forceinline__ void sincos_small(float v, float &s, float &
     c = fmaf(v2, pc, 1.0f);
 }
 
+// sin v and cos v - 1 (the latter without the cancellation of cosf(v) - 1),
+// same polynomial as sincos_small.
+__host__ __device__ __forceinline__ void sincosm1_small(float v, float &s, float &cm1) {
+    if (!(fabsf(v) <= 1.0f)) {
+        s = sinf(v);
+        cm1 = cosf(v) - 1.0f;
+        return;
+    }
+    const float v2 = v * v;
+    float ps = -2.50521083854417e-08f;
+    ps = fmaf(ps, v2, 2.75573192239859e-06f);
+    ps = fmaf(ps, v2, -1.98412698412698e-04f);
+    ps = fmaf(ps, v2, 8.33333333333333e-03f);
+    ps = fmaf(ps, v2, -1.66666666666667e-01f);
+    s = fmaf(v * v2, ps, v);
+    float pc = 2.08767569878681e-09f;
+    pc = fmaf(pc, v2, -2.75573192239859e-07f);
+    pc = fmaf(pc, v2, 2.48015873015873e-05f);
+    pc = fmaf(pc, v2, -1.38888888888889e-03f);
+    pc = fmaf(pc, v2, 4.16666666666667e-02f);
+    pc = fmaf(pc, v2, -0.5f);
+    cm1 = v2 * pc;
+}
+
 __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << __lane_id()) - 1ull;
 }

@@ -164,8 +164,8 @@ __device__ __forceinline__ bool cell_in_window(const int *__restrict__ cell_g, i
     return in && !*stop;
 }
 
-// A source run of a cell staged in LDS: its position, the sine / cosine of
-// its offset from the tile's cell corner, and its length.
+// A source run of a cell staged in LDS: its position and, for its offset v
+// from the tile's anchor, sin v and cos v - 1 (in cv).
 template <int D>
 struct SrcPt {
     static constexpr int P = (D + 3) / 4 * 4;
@@ -243,13 +243,16 @@ __device__ __forceinline__ int next_tile(int *counter) {
 
 // kernel_itr_grid_STAD5_5_1 (EGG_SynC.cu:551-763).  Per lane: full cells add
 // the cell totals (EGG_SynC.cu:648-663); partial cells are streamed through
-// LDS in chunks of 64 points -- each lane loads one point and computes sin /
-// cos of v = x_q - a0 (a0 = the tile's cell corner) -- and every lane reads
-// each point as a broadcast and, for points within eps, adds
-// sin(x_q - x) = sin(v) cos(u) - cos(v) sin(u), u = x - a0.  Both angles are
-// below eps + 2 cell sides, so each term keeps an error of a few ulps of eps,
-// like a direct sinf(x_q - x).  Border lanes count their eps/2-neighbours in
-// other full cells from the same chunks (EGG_SynC.cu:666-681).
+// LDS in chunks of 64 runs -- each lane loads one run and computes sin v and
+// cos v - 1 of v = x_q - a0 (a0 = the tile's anchor, near the tile's cells) --
+// and every lane reads each run as a broadcast and, for runs within eps,
+// accumulates sin v and cos v - 1 (once per point of the run).  At the end of
+// the tile, sum_q sin(x_q - x) = cos u * sum sin v - sin u * (m + sum (cos v - 1)),
+// u = x - a0, m = number of such points.  All angles stay within a few cell
+// sides of the anchor, and both sums grow like the sum of sin(x_q - x) does,
+// so the result keeps the accuracy of a direct sum of sinf(x_q - x).  Border
+// lanes count their eps/2-neighbours in other full cells from the same
+// chunks (EGG_SynC.cu:666-681).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_update_tile(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
@@ -277,19 +280,21 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         const bool active = tl.active;
         const int own = cell_start[cl + 1] - cell_start[cl];
         const int ghi = cell_g[(long long)(D - 1) * n + tl.gc1 - 1];
-        float x[D], a0[D], sx[D], cx[D], su[D], cu[D], sum[D], sml[D];
+        float x[D], a0[D], sx[D], cx[D], su[D], cu[D], sum[D], ssv[D], scm[D];
         int g0[D];
         bool border = false;
 #pragma unroll
         for (int l = 0; l < D; l++) {
             g0[l] = cell_g[(long long)l * n + c0];
-            // anchor: the corner of the tile's cell in the outer dimensions
+            // anchor: the centre of the tile's cell in the outer dimensions
             // and of its block of kAnchorBlock cells in the inner ones, the
             // same for every group width that divides kAnchorBlock
-            a0[l] = (float)(l < od ? g0[l] : g0[l] & ~(kAnchorBlock - 1)) * g.cs;
+            a0[l] = l < od ? ((float)g0[l] + 0.5f) * g.cs
+                           : ((float)(g0[l] & ~(kAnchorBlock - 1)) + 0.5f * kAnchorBlock) * g.cs;
             x[l] = active ? Xs[(long long)l * n + i] : a0[l];
             sum[l] = 0.f;
-            sml[l] = 0.f;
+            ssv[l] = 0.f;
+            scm[l] = 0.f;
             sx[l] = sinf(x[l]);
             cx[l] = cosf(x[l]);
             const float u = x[l] - a0[l];
@@ -297,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
             border |= active && on_border(x[l], g.cs);
         }
         const int g0od = select_dim<D>(g0, od);
-        int N = 0, half = 0;
+        int N = 0, half = 0, m_part = 0;
         walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
             if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
             for (int c = cb; c < ce; c++) {
@@ -343,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                             src[lane].x[l] = xq;
                             if (any_part) {
                                 const float v = xq - a0[l];
-                                sincos_small(v, src[lane].sv[l], src[lane].cv[l]);
+                                sincosm1_small(v, src[lane].sv[l], src[lane].cv[l]);
                             }
                         }
                     }
@@ -351,9 +356,36 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                     wave_sync();
                     const int m = min(kWave, r1 - base);
                     if (part && !multi) {
-                        // every run of the chunk is a single point
-#pragma unroll 4
-                        for (int k = 0; k < m; k++) {
+                        // every run of the chunk is a single point; four
+                        // sources per step so their LDS reads overlap
+                        int k = 0;
+                        for (; k + 4 <= m; k += 4) {
+                            float s[4];
+#pragma unroll
+                            for (int j = 0; j < 4; j++) {
+                                float q[D];
+#pragma unroll
+                                for (int l = 0; l < D; l++) q[l] = src[k + j].x[l];
+                                s[j] = 0.f;
+#pragma unroll
+                                for (int l = 0; l < D; l++) {
+                                    const float df = x[l] - q[l];
+                                    s[j] += df * df;
+                                }
+                            }
+#pragma unroll
+                            for (int j = 0; j < 4; j++) {
+                                if (s[j] <= thr) {
+#pragma unroll
+                                    for (int l = 0; l < D; l++) {
+                                        ssv[l] += src[k + j].sv[l];
+                                        scm[l] += src[k + j].cv[l];
+                                    }
+                                    m_part += 1;
+                                }
+                            }
+                        }
+                        for (; k < m; k++) {
                             const SrcPt<D> &q = src[k];
                             float s = 0.f;
 #pragma unroll
@@ -364,15 +396,13 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                             if (s <= thr) {
 #pragma unroll
                                 for (int l = 0; l < D; l++) {
-                                    const float p1 = q.sv[l] * cu[l];
-                                    const float p2 = q.cv[l] * su[l];
-                                    sml[l] += p1 - p2;
+                                    ssv[l] += q.sv[l];
+                                    scm[l] += q.cv[l];
                                 }
-                                N += 1;
+                                m_part += 1;
                             }
                         }
                     } else if (part) {
-#pragma unroll 2
                         for (int k = 0; k < m; k++) {
                             const SrcPt<D> &q = src[k];
                             float s = 0.f;
@@ -385,12 +415,13 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                                 const int lk = len[k];
 #pragma unroll
                                 for (int l = 0; l < D; l++) {
-                                    const float p1 = q.sv[l] * cu[l];
-                                    const float p2 = q.cv[l] * su[l];
-                                    const float tt = p1 - p2;
-                                    for (int e = 0; e < lk; e++) sml[l] += tt;
+                                    const float a = q.sv[l], b = q.cv[l];
+                                    for (int e = 0; e < lk; e++) {
+                                        ssv[l] += a;
+                                        scm[l] += b;
+                                    }
                                 }
-                                N += lk;
+                                m_part += lk;
                             }
                         }
                     } else if (need_half) {
@@ -408,10 +439,14 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                 }
             }
         });
+        N += m_part;
         float xo[D];
 #pragma unroll
         for (int l = 0; l < D; l++) {
-            float tt = sum[l] + sml[l];
+            const float p1 = ssv[l] * cu[l];
+            const float p2 = ((float)m_part + scm[l]) * su[l];
+            const float sml = p1 - p2;
+            float tt = sum[l] + sml;
             tt = tt / (float)N;
             xo[l] = x[l] + tt;
         }
