@@ -191,22 +191,29 @@ __device__ __forceinline__ int tile_group(const int *__restrict__ tile_off, int 
     return lo;
 }
 
-// A tile: its group (cells [gc0, gc1), all in one outer cell, equal in every
-// inner coordinate but the last) and its runs [rbase, rend), at most kTile.
-struct Tile {
-    int gc0, gc1, rbase, rend;
+// A tile seen by one lane: the tile's group (cells [gc0, gc1), all in one
+// outer cell, equal in every inner coordinate but the last), the lane's run
+// r (active if it belongs to the tile), its first sorted position i and its
+// cell cl.
+struct TileLane {
+    int gc0, gc1, r, i, cl;
+    bool active;
 };
 
-__device__ __forceinline__ Tile tile_at(int t, const int *__restrict__ tile_off, int G,
-                                        const int *__restrict__ grp_cell,
-                                        const int *__restrict__ cell_run) {
-    Tile T;
+__device__ __forceinline__ TileLane tile_lane(int t, const int *__restrict__ tile_off, int G,
+                                              const int *__restrict__ grp_cell,
+                                              const int *__restrict__ cell_run,
+                                              const int *__restrict__ run_start,
+                                              const int *__restrict__ point_cell) {
+    TileLane L;
     const int k = tile_group(tile_off, G, t);
-    T.gc0 = grp_cell[k];
-    T.gc1 = grp_cell[k + 1];
-    T.rbase = cell_run[T.gc0] + (t - tile_off[k]) * kTile;
-    T.rend = min(cell_run[T.gc1], T.rbase + kTile);
-    return T;
+    L.gc0 = grp_cell[k];
+    L.gc1 = grp_cell[k + 1];
+    L.r = cell_run[L.gc0] + (t - tile_off[k]) * kTile + __lane_id();
+    L.active = L.r < cell_run[L.gc1];
+    L.i = L.active ? run_start[L.r] : 0;
+    L.cl = L.active ? point_cell[L.i] : L.gc0;
+    return L;
 }
 
 // cell_in_window for a group: the window of the last dimension spans
@@ -246,231 +253,6 @@ __device__ __forceinline__ int next_tile(int *counter) {
 // so the result keeps the accuracy of a direct sum of sinf(x_q - x).  Border
 // lanes count their eps/2-neighbours in other full cells from the same
 // chunks (EGG_SynC.cu:666-681).
-// One tile of the update, NT targets per lane: runs rbase + z*64 + lane,
-// z < NT, of the group [gc0, gc1).  Every source a wavefront stages in LDS is
-// tested against all NT targets of every lane, so with NT = 2 one LDS
-// broadcast serves 128 targets.
-template <int D, int NT>
-__device__ __forceinline__ void update_tile(
-    int gc0, int gc1, int rbase, int rend, const float *__restrict__ Xs,
-    const int *__restrict__ cell_start, const int *__restrict__ cell_g,
-    const float *__restrict__ Ts, const float *__restrict__ Tc, const int *__restrict__ ob,
-    const int *__restrict__ oe, const int *__restrict__ run_start,
-    const int *__restrict__ cell_run, const int *__restrict__ point_cell,
-    float *__restrict__ rres, int *__restrict__ rdbg, const Geo &g, SrcPt<D> *src, int *len,
-    bool &unsynced) {
-    const int n = g.n, R = g.R, od = g.od;
-    const float thr = g.thr_eps, thr_half = g.thr_half;
-    const int lane = __lane_id();
-    const int *cg_od = cell_g + (long long)od * n;
-    const int ghi = cell_g[(long long)(D - 1) * n + gc1 - 1];
-    int g0[D];
-    float a0[D];
-#pragma unroll
-    for (int l = 0; l < D; l++) {
-        g0[l] = cell_g[(long long)l * n + gc0];
-        // anchor: the centre of the tile's cell in the outer dimensions and
-        // of its block of kAnchorBlock cells in the inner ones, the same for
-        // every group width that divides kAnchorBlock
-        a0[l] = l < od ? ((float)g0[l] + 0.5f) * g.cs
-                       : ((float)(g0[l] & ~(kAnchorBlock - 1)) + 0.5f * kAnchorBlock) * g.cs;
-    }
-    float x[NT][D], sx[NT][D], cx[NT][D], su[NT][D], cu[NT][D], sum[NT][D], ssv[NT][D],
-        scm[NT][D];
-    int r[NT], cl[NT], own[NT], N[NT], half[NT], mp[NT];
-    bool active[NT], border[NT];
-#pragma unroll
-    for (int z = 0; z < NT; z++) {
-        r[z] = rbase + z * kWave + lane;
-        active[z] = r[z] < rend;
-        const int i = active[z] ? run_start[r[z]] : 0;
-        cl[z] = active[z] ? point_cell[i] : gc0;
-        own[z] = cell_start[cl[z] + 1] - cell_start[cl[z]];
-        N[z] = half[z] = mp[z] = 0;
-        border[z] = false;
-#pragma unroll
-        for (int l = 0; l < D; l++) {
-            x[z][l] = active[z] ? Xs[(long long)l * n + i] : a0[l];
-            sum[z][l] = ssv[z][l] = scm[z][l] = 0.f;
-            sx[z][l] = sinf(x[z][l]);
-            cx[z][l] = cosf(x[z][l]);
-            sincos_small(x[z][l] - a0[l], su[z][l], cu[z][l]);
-            border[z] = border[z] || (active[z] && on_border(x[z][l], g.cs));
-        }
-    }
-    const int g0od = select_dim<D>(g0, od);
-    walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
-        if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
-        for (int c = cb; c < ce; c++) {
-            int gc[D];
-            bool stop;
-            const bool in = cell_in_group_window<D>(cell_g, n, c, g0, od, R + 1, ghi, gc, &stop);
-            if (stop) break;
-            if (!in) continue;
-            bool part[NT], need_half[NT], lane_part = false, lane_half = false;
-#pragma unroll
-            for (int z = 0; z < NT; z++) {
-                float sm, lg;
-                box_dist<D>(gc, x[z], g.cs, sm, lg);
-                const bool full = active[z] && lg <= thr;
-                part[z] = active[z] && !full && sm <= thr;
-                if (full) {
-#pragma unroll
-                    for (int l = 0; l < D; l++) {
-                        const float a = Ts[(long long)l * n + c] * cx[z][l];
-                        const float b = Tc[(long long)l * n + c] * sx[z][l];
-                        const float tt = a - b;
-                        sum[z][l] += tt;
-                    }
-                    const int cnt = cell_start[c + 1] - cell_start[c];
-                    N[z] += cnt;
-                    if (border[z] && c == cl[z]) half[z] += cnt;
-                }
-                need_half[z] = full && border[z] && c != cl[z];
-                lane_part = lane_part || part[z];
-                lane_half = lane_half || need_half[z];
-            }
-            const bool any_part = __any(lane_part), any_half = __any(lane_half);
-            if (!any_part && !any_half) continue;
-            // the cell's runs, 64 per chunk: one test per run and target,
-            // each counted with its length; a run's term is added once per
-            // point, in order, as a pointwise loop would
-            const int r0 = cell_run[c], r1 = cell_run[c + 1];
-            for (int base = r0; base < r1; base += kWave) {
-                const int rr = base + lane;
-                int my_len = 1;
-                if (rr < r1) {
-                    const int j = run_start[rr];
-                    my_len = run_start[rr + 1] - j;
-                    len[lane] = my_len;
-#pragma unroll
-                    for (int l = 0; l < D; l++) {
-                        const float xq = Xs[(long long)l * n + j];
-                        src[lane].x[l] = xq;
-                        if (any_part) sincosm1_small(xq - a0[l], src[lane].sv[l], src[lane].cv[l]);
-                    }
-                }
-                const bool multi = __any(my_len > 1);
-                wave_sync();
-                const int m = min(kWave, r1 - base);
-                if (!multi && !any_half) {
-                    if (lane_part) {
-                        // single-point runs, no eps/2 counting: four sources
-                        // per step so their LDS reads overlap
-                        int k = 0;
-                        for (; k + 4 <= m; k += 4) {
-                            float s[4][NT];
-#pragma unroll
-                            for (int j = 0; j < 4; j++) {
-                                float q[D];
-#pragma unroll
-                                for (int l = 0; l < D; l++) q[l] = src[k + j].x[l];
-#pragma unroll
-                                for (int z = 0; z < NT; z++) {
-                                    s[j][z] = 0.f;
-#pragma unroll
-                                    for (int l = 0; l < D; l++) {
-                                        const float df = x[z][l] - q[l];
-                                        s[j][z] += df * df;
-                                    }
-                                }
-                            }
-#pragma unroll
-                            for (int j = 0; j < 4; j++) {
-#pragma unroll
-                                for (int z = 0; z < NT; z++) {
-                                    if (part[z] && s[j][z] <= thr) {
-#pragma unroll
-                                        for (int l = 0; l < D; l++) {
-                                            ssv[z][l] += src[k + j].sv[l];
-                                            scm[z][l] += src[k + j].cv[l];
-                                        }
-                                        mp[z] += 1;
-                                    }
-                                }
-                            }
-                        }
-                        for (; k < m; k++) {
-                            const SrcPt<D> &q = src[k];
-#pragma unroll
-                            for (int z = 0; z < NT; z++) {
-                                float s = 0.f;
-#pragma unroll
-                                for (int l = 0; l < D; l++) {
-                                    const float df = x[z][l] - q.x[l];
-                                    s += df * df;
-                                }
-                                if (part[z] && s <= thr) {
-#pragma unroll
-                                    for (int l = 0; l < D; l++) {
-                                        ssv[z][l] += q.sv[l];
-                                        scm[z][l] += q.cv[l];
-                                    }
-                                    mp[z] += 1;
-                                }
-                            }
-                        }
-                    }
-                } else if (lane_part || lane_half) {
-                    for (int k = 0; k < m; k++) {
-                        const SrcPt<D> &q = src[k];
-                        const int lk = len[k];
-#pragma unroll
-                        for (int z = 0; z < NT; z++) {
-                            float s = 0.f;
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                const float df = x[z][l] - q.x[l];
-                                s += df * df;
-                            }
-                            if (part[z] && s <= thr) {
-#pragma unroll
-                                for (int l = 0; l < D; l++) {
-                                    const float a = q.sv[l], b = q.cv[l];
-                                    for (int e = 0; e < lk; e++) {
-                                        ssv[z][l] += a;
-                                        scm[z][l] += b;
-                                    }
-                                }
-                                mp[z] += lk;
-                            } else if (need_half[z] && s <= thr_half) {
-                                half[z] += lk;
-                            }
-                        }
-                    }
-                }
-                wave_sync();
-            }
-        }
-    });
-#pragma unroll
-    for (int z = 0; z < NT; z++) {
-        const int Nz = N[z] + mp[z];
-        float xo[D];
-#pragma unroll
-        for (int l = 0; l < D; l++) {
-            const float p1 = ssv[z][l] * cu[z][l];
-            const float p2 = ((float)mp[z] + scm[z][l]) * su[z][l];
-            const float sml = p1 - p2;
-            float tt = sum[z][l] + sml;
-            tt = tt / (float)Nz;
-            xo[l] = x[z][l] + tt;
-        }
-        const bool synced = !((!border[z] && Nz != own[z]) || (border[z] && Nz != half[z]));
-        if (active[z]) {
-#pragma unroll
-            for (int l = 0; l < D; l++) rres[(long long)l * n + r[z]] = xo[l];
-            if (!synced) unsynced = true;
-            if (rdbg) {
-                rdbg[4 * (long long)r[z] + 0] = Nz;
-                rdbg[4 * (long long)r[z] + 1] = half[z];
-                rdbg[4 * (long long)r[z] + 2] = border[z] ? 1 : 0;
-                rdbg[4 * (long long)r[z] + 3] = synced ? 1 : 0;
-            }
-        }
-    }
-}
-
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_update_tile(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
@@ -482,25 +264,208 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     if (st->done) return;
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
     __shared__ int s_len[kBlock / kWave][kWave];
+    const int n = g.n, R = g.R, od = g.od;
     const int G = st->G, T = st->T;
+    const float thr = g.thr_eps, thr_half = g.thr_half;
+    const int lane = __lane_id();
     SrcPt<D> *src = s_src[threadIdx.x / kWave];
     int *len = s_len[threadIdx.x / kWave];
+    const int *cg_od = cell_g + (long long)od * n;
     bool unsynced = false;
     for (;;) {
         const int t = next_tile(&st->tile_next);
         if (t >= T) break;
-        const Tile tl = tile_at(t, tile_off, G, grp_cell, cell_run);
-        if (tl.rend - tl.rbase > kWave)
-            update_tile<D, 2>(tl.gc0, tl.gc1, tl.rbase, tl.rend, Xs, cell_start, cell_g, Ts, Tc,
-                              ob, oe, run_start, cell_run, point_cell, rres, rdbg, g, src, len,
-                              unsynced);
-        else
-            update_tile<D, 1>(tl.gc0, tl.gc1, tl.rbase, tl.rend, Xs, cell_start, cell_g, Ts, Tc,
-                              ob, oe, run_start, cell_run, point_cell, rres, rdbg, g, src, len,
-                              unsynced);
+        const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
+        const int c0 = tl.gc0, cl = tl.cl, r = tl.r, i = tl.i;
+        const bool active = tl.active;
+        const int own = cell_start[cl + 1] - cell_start[cl];
+        const int ghi = cell_g[(long long)(D - 1) * n + tl.gc1 - 1];
+        float x[D], a0[D], sx[D], cx[D], su[D], cu[D], sum[D], ssv[D], scm[D];
+        int g0[D];
+        bool border = false;
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            g0[l] = cell_g[(long long)l * n + c0];
+            // anchor: the centre of the tile's cell in the outer dimensions
+            // and of its block of kAnchorBlock cells in the inner ones, the
+            // same for every group width that divides kAnchorBlock
+            a0[l] = l < od ? ((float)g0[l] + 0.5f) * g.cs
+                           : ((float)(g0[l] & ~(kAnchorBlock - 1)) + 0.5f * kAnchorBlock) * g.cs;
+            x[l] = active ? Xs[(long long)l * n + i] : a0[l];
+            sum[l] = 0.f;
+            ssv[l] = 0.f;
+            scm[l] = 0.f;
+            sx[l] = sinf(x[l]);
+            cx[l] = cosf(x[l]);
+            const float u = x[l] - a0[l];
+            sincos_small(u, su[l], cu[l]);
+            border |= active && on_border(x[l], g.cs);
+        }
+        const int g0od = select_dim<D>(g0, od);
+        int N = 0, half = 0, m_part = 0;
+        walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
+            if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+            for (int c = cb; c < ce; c++) {
+                int gc[D];
+                bool stop;
+                const bool in =
+                    cell_in_group_window<D>(cell_g, n, c, g0, od, R + 1, ghi, gc, &stop);
+                if (stop) break;
+                if (!in) continue;
+                float sm, lg;
+                box_dist<D>(gc, x, g.cs, sm, lg);
+                const bool full = active && lg <= thr;
+                const bool part = active && !full && sm <= thr;
+                if (full) {
+#pragma unroll
+                    for (int l = 0; l < D; l++) {
+                        const float a = Ts[(long long)l * n + c] * cx[l];
+                        const float b = Tc[(long long)l * n + c] * sx[l];
+                        const float tt = a - b;
+                        sum[l] += tt;
+                    }
+                    const int cnt = cell_start[c + 1] - cell_start[c];
+                    N += cnt;
+                    if (border && c == cl) half += cnt;
+                }
+                const bool need_half = full && border && c != cl;
+                const bool any_part = __any(part);
+                if (!any_part && !__any(need_half)) continue;
+                // the cell's runs, 64 per chunk: one test per run, each
+                // counted with its length; a run's term is added once per
+                // point, in order, as a pointwise loop would
+                const int r0 = cell_run[c], r1 = cell_run[c + 1];
+                for (int base = r0; base < r1; base += kWave) {
+                    const int rr = base + lane;
+                    int my_len = 1;
+                    if (rr < r1) {
+                        const int j = run_start[rr];
+                        my_len = run_start[rr + 1] - j;
+                        len[lane] = my_len;
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            const float xq = Xs[(long long)l * n + j];
+                            src[lane].x[l] = xq;
+                            if (any_part) {
+                                const float v = xq - a0[l];
+                                sincosm1_small(v, src[lane].sv[l], src[lane].cv[l]);
+                            }
+                        }
+                    }
+                    const bool multi = __any(my_len > 1);
+                    wave_sync();
+                    const int m = min(kWave, r1 - base);
+                    if (part && !multi) {
+                        // every run of the chunk is a single point; four
+                        // sources per step so their LDS reads overlap
+                        int k = 0;
+                        for (; k + 4 <= m; k += 4) {
+                            float s[4];
+#pragma unroll
+                            for (int j = 0; j < 4; j++) {
+                                float q[D];
+#pragma unroll
+                                for (int l = 0; l < D; l++) q[l] = src[k + j].x[l];
+                                s[j] = 0.f;
+#pragma unroll
+                                for (int l = 0; l < D; l++) {
+                                    const float df = x[l] - q[l];
+                                    s[j] += df * df;
+                                }
+                            }
+#pragma unroll
+                            for (int j = 0; j < 4; j++) {
+                                if (s[j] <= thr) {
+#pragma unroll
+                                    for (int l = 0; l < D; l++) {
+                                        ssv[l] += src[k + j].sv[l];
+                                        scm[l] += src[k + j].cv[l];
+                                    }
+                                    m_part += 1;
+                                }
+                            }
+                        }
+                        for (; k < m; k++) {
+                            const SrcPt<D> &q = src[k];
+                            float s = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float df = x[l] - q.x[l];
+                                s += df * df;
+                            }
+                            if (s <= thr) {
+#pragma unroll
+                                for (int l = 0; l < D; l++) {
+                                    ssv[l] += q.sv[l];
+                                    scm[l] += q.cv[l];
+                                }
+                                m_part += 1;
+                            }
+                        }
+                    } else if (part) {
+                        for (int k = 0; k < m; k++) {
+                            const SrcPt<D> &q = src[k];
+                            float s = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float df = x[l] - q.x[l];
+                                s += df * df;
+                            }
+                            if (s <= thr) {
+                                const int lk = len[k];
+#pragma unroll
+                                for (int l = 0; l < D; l++) {
+                                    const float a = q.sv[l], b = q.cv[l];
+                                    for (int e = 0; e < lk; e++) {
+                                        ssv[l] += a;
+                                        scm[l] += b;
+                                    }
+                                }
+                                m_part += lk;
+                            }
+                        }
+                    } else if (need_half) {
+                        for (int k = 0; k < m; k++) {
+                            float s = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float df = x[l] - src[k].x[l];
+                                s += df * df;
+                            }
+                            if (s <= thr_half) half += len[k];
+                        }
+                    }
+                    wave_sync();
+                }
+            }
+        });
+        N += m_part;
+        float xo[D];
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            const float p1 = ssv[l] * cu[l];
+            const float p2 = ((float)m_part + scm[l]) * su[l];
+            const float sml = p1 - p2;
+            float tt = sum[l] + sml;
+            tt = tt / (float)N;
+            xo[l] = x[l] + tt;
+        }
+        const bool synced = !((!border && N != own) || (border && N != half));
+        if (active) {
+#pragma unroll
+            for (int l = 0; l < D; l++) rres[(long long)l * n + r] = xo[l];
+            if (!synced) unsynced = true;
+            if (rdbg) {
+                rdbg[4 * (long long)r + 0] = N;
+                rdbg[4 * (long long)r + 1] = half;
+                rdbg[4 * (long long)r + 2] = border ? 1 : 0;
+                rdbg[4 * (long long)r + 3] = synced ? 1 : 0;
+            }
+        }
     }
-    if (__any(unsynced) && __lane_id() == 0) atomicAnd(&st->flag, 0);
+    if (__any(unsynced) && lane == 0) atomicAnd(&st->flag, 0);
 }
+
 // Every point takes its run's result and carries its id over to the next
 // iteration's order.  The run of sorted position i is hpos[i] + head[i] - 1.
 template <int D>
@@ -615,80 +580,76 @@ __global__ __launch_bounds__(kBlock) void k_extra_tile(
         const int t = next_tile(&st->xtile_next);
         if (t >= T) break;
         if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-        const Tile tl = tile_at(t, tile_off, G, grp_cell, cell_run);
-        const int c0 = tl.gc0;
+        const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
+        const int c0 = tl.gc0, i = tl.i;
+        const bool active = tl.active;
         const int ghi = cell_g[(long long)(D - 1) * n + tl.gc1 - 1];
-        for (int half_base = tl.rbase; half_base < tl.rend; half_base += kWave) {
-            const int r = half_base + lane;
-            const bool active = r < tl.rend;
-            const int i = active ? run_start[r] : 0;
-            float xp[D];
-            int g0[D];
+        float xp[D];
+        int g0[D];
 #pragma unroll
-            for (int l = 0; l < D; l++) {
-                g0[l] = cell_g[(long long)l * n + c0];
-                xp[l] = active ? Xs[(long long)l * n + i] : 0.f;
-            }
-            float lq[D];
-            bool have_lq = false;
-            const int g0od = select_dim<D>(g0, od);
-            auto consider = [&](const float *xq, int j) {
-                bool rep = have_lq;
-#pragma unroll
-                for (int l = 0; l < D; l++) rep = rep && __float_as_uint(lq[l]) == __float_as_uint(xq[l]);
-                if (rep) return;
-#pragma unroll
-                for (int l = 0; l < D; l++) lq[l] = xq[l];
-                have_lq = true;
-                const int slot = atomicAdd(&st->pair_count, 1);
-                if (slot < pair_cap) {
-                    pairs[2 * slot] = i;
-                    pairs[2 * slot + 1] = j;
-                } else if (mbr_in_eps<D>(xp, xq, Xs, cell_g, ob, oe, run_start, cell_run, g)) {
-                    hit = true;
-                }
-            };
-            walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
-                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
-                for (int c = cb; c < ce; c++) {
-                    int gc[D];
-                    bool stop;
-                    const bool in =
-                        cell_in_group_window<D>(cell_g, n, c, g0, od, R + 1, ghi, gc, &stop);
-                    if (stop) break;
-                    if (!in) continue;
-                    float sm, lg;
-                    box_dist<D>(gc, xp, g.cs, sm, lg);
-                    const bool need = active && !(lg <= g.thr_eps) && sm <= g.thr_extra;
-                    if (!__any(need)) continue;
-                    const int r0 = cell_run[c], r1 = cell_run[c + 1];
-                    for (int base = r0; base < r1; base += kWave) {
-                        const int rr = base + lane;
-                        if (rr < r1) {
-                            const int j = run_start[rr];
-                            src_pos[lane] = j;
-#pragma unroll
-                            for (int l = 0; l < D; l++) src[lane][l] = Xs[(long long)l * n + j];
-                        }
-                        wave_sync();
-                        const int m = min(kWave, r1 - base);
-                        if (need) {
-                            for (int k = 0; k < m; k++) {
-                                float xq[D], s = 0.f;
-#pragma unroll
-                                for (int l = 0; l < D; l++) {
-                                    xq[l] = src[k][l];
-                                    const float df = xp[l] - xq[l];
-                                    s += df * df;
-                                }
-                                if (!(s <= g.thr_eps) && s <= g.thr_extra) consider(xq, src_pos[k]);
-                            }
-                        }
-                        wave_sync();
-                    }
-                }
-            });
+        for (int l = 0; l < D; l++) {
+            g0[l] = cell_g[(long long)l * n + c0];
+            xp[l] = active ? Xs[(long long)l * n + i] : 0.f;
         }
+        float lq[D];
+        bool have_lq = false;
+        const int g0od = select_dim<D>(g0, od);
+        auto consider = [&](const float *xq, int j) {
+            bool rep = have_lq;
+#pragma unroll
+            for (int l = 0; l < D; l++) rep = rep && __float_as_uint(lq[l]) == __float_as_uint(xq[l]);
+            if (rep) return;
+#pragma unroll
+            for (int l = 0; l < D; l++) lq[l] = xq[l];
+            have_lq = true;
+            const int slot = atomicAdd(&st->pair_count, 1);
+            if (slot < pair_cap) {
+                pairs[2 * slot] = i;
+                pairs[2 * slot + 1] = j;
+            } else if (mbr_in_eps<D>(xp, xq, Xs, cell_g, ob, oe, run_start, cell_run, g)) {
+                hit = true;
+            }
+        };
+        walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
+            if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+            for (int c = cb; c < ce; c++) {
+                int gc[D];
+                bool stop;
+                const bool in =
+                    cell_in_group_window<D>(cell_g, n, c, g0, od, R + 1, ghi, gc, &stop);
+                if (stop) break;
+                if (!in) continue;
+                float sm, lg;
+                box_dist<D>(gc, xp, g.cs, sm, lg);
+                const bool need = active && !(lg <= g.thr_eps) && sm <= g.thr_extra;
+                if (!__any(need)) continue;
+                const int r0 = cell_run[c], r1 = cell_run[c + 1];
+                for (int base = r0; base < r1; base += kWave) {
+                    const int rr = base + lane;
+                    if (rr < r1) {
+                        const int j = run_start[rr];
+                        src_pos[lane] = j;
+#pragma unroll
+                        for (int l = 0; l < D; l++) src[lane][l] = Xs[(long long)l * n + j];
+                    }
+                    wave_sync();
+                    const int m = min(kWave, r1 - base);
+                    if (need) {
+                        for (int k = 0; k < m; k++) {
+                            float xq[D], s = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                xq[l] = src[k][l];
+                                const float df = xp[l] - xq[l];
+                                s += df * df;
+                            }
+                            if (!(s <= g.thr_eps) && s <= g.thr_extra) consider(xq, src_pos[k]);
+                        }
+                    }
+                    wave_sync();
+                }
+            }
+        });
     }
     if (__any(hit) && lane == 0) atomicAnd(&st->flag, 0);
 }
