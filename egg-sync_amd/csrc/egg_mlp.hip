@@ -319,17 +319,21 @@ int egg_mlp_train_step_device(float *d_params, float *d_adam_m, float *d_adam_v,
 
 // x <- clamp(x + scale * y, 0, 1) for n x d row-major positions.
 __global__ void k_jump(float *__restrict__ X, const float *__restrict__ Y, long long count,
-                       float scale) {
+                       float scale, float clip) {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
-         e += (long long)gridDim.x * blockDim.x)
-        X[e] = fminf(fmaxf(X[e] + scale * Y[e], 0.f), 1.f);
+         e += (long long)gridDim.x * blockDim.x) {
+        float y = Y[e];
+        if (clip > 0.f) y = fminf(fmaxf(y, -clip), clip);
+        X[e] = fminf(fmaxf(X[e] + scale * y, 0.f), 1.f);
+    }
 }
 
 int egg_sync_jump_device(float *d_X, const float *d_Y, int64_t n, int32_t d, float scale,
-                         void *stream) {
+                         float clip, void *stream) {
     if ((!d_X || !d_Y) && n > 0) return EGG_SYNC_E_ARG;
     if (n == 0) return EGG_SYNC_OK;
-    k_jump<<<blocks_for(n * d), kThreads, 0, (hipStream_t)stream>>>(d_X, d_Y, n * (long long)d, scale);
+    k_jump<<<blocks_for(n * d), kThreads, 0, (hipStream_t)stream>>>(d_X, d_Y, n * (long long)d,
+                                                                     scale, clip);
     return hipGetLastError() == hipSuccess ? EGG_SYNC_OK : EGG_SYNC_E_HIP;
 }
 

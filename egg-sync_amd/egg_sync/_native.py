@@ -94,7 +94,7 @@ def lib():
                                         ctypes.POINTER(ctypes.c_float)]
         L.egg_sync_release.argtypes = []
         L.egg_sync_features_device.argtypes = [P, P, i64, i32, f32, P, P]
-        L.egg_sync_jump_device.argtypes = [P, P, i64, i32, f32, P]
+        L.egg_sync_jump_device.argtypes = [P, P, i64, i32, f32, f32, P]
         L.egg_mlp_param_count.argtypes = [i32, i32, i32, ctypes.POINTER(ctypes.c_int64)]
         L.egg_mlp_workspace_floats.argtypes = [i32, i32, i32, i64]
         L.egg_mlp_workspace_floats.restype = ctypes.c_size_t

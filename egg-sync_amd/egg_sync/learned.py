@@ -156,8 +156,12 @@ def trajectory_samples(X: np.ndarray, eps: float, t0: int = 1):
     return F, T
 
 
-def solve_learned(X, eps: float, mlp: JumpMLP, t0: int = 1, verify: bool = False):
-    """t0 plain passes, one learned jump, plain passes until EGG-SynC's stop rule."""
+def solve_learned(X, eps: float, mlp: JumpMLP, t0: int = 1, verify: bool = False,
+                  alpha: float = 1.0, clip: float = 0.0):
+    """t0 plain passes, one learned jump, plain passes until EGG-SynC's stop rule.
+
+    The jump moves each point by alpha * eps * (predicted displacement / eps),
+    each component first clamped to [-clip, clip] (no clamp if clip <= 0)."""
     D = solver._dev_f32(X)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
@@ -166,7 +170,8 @@ def solve_learned(X, eps: float, mlp: JumpMLP, t0: int = 1, verify: bool = False
     Y = mlp.forward(F)
     jumped = cur.clone()
     nat.check(nat.lib().egg_sync_jump_device(nat.ptr(jumped), nat.ptr(Y), jumped.shape[0],
-                                             jumped.shape[1], float(eps), _stream()),
+                                             jumped.shape[1], float(alpha * eps), float(clip),
+                                             _stream()),
               "egg_sync_jump_device")
     r = solver.solve(jumped, eps)
     torch.cuda.synchronize()

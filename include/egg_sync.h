@@ -179,9 +179,10 @@ int egg_sync_rc_device(const float *d_D, int64_t n, int32_t d, float eps, double
 int egg_sync_features_device(const float *d_D, const float *d_prev, int64_t n, int32_t d,
                              float eps, float *d_feat, void *stream);
 
-/* d_X <- clamp(d_X + scale * d_Y, 0, 1), n x d. */
+/* d_X <- clamp(d_X + scale * clamp(d_Y, -clip, clip), 0, 1), n x d (no clamp of
+ * d_Y if clip <= 0). */
 int egg_sync_jump_device(float *d_X, const float *d_Y, int64_t n, int32_t d, float scale,
-                         void *stream);
+                         float clip, void *stream);
 
 /* Number of parameters of an in-hidden-hidden-out MLP. */
 int egg_mlp_param_count(int32_t in, int32_t hidden, int32_t out, int64_t *count);
