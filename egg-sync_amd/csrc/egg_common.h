@@ -48,13 +48,24 @@ struct Geo {
     float thr_extra;   // largest s with sqrtf(s) <= eps_extra
     int bits_w;        // bits of one coordinate field in the key
     int bits_o;        // bits of the outer id field
-    int key_bits;      // total key bits
+    int key_bits;      // bits of the cell part of the key
+    int sub_bits;      // bits of the sub-cell part below it (d, or 0 if keys would exceed 128 bits)
+    int sort_bits;     // key_bits + sub_bits
     int kw;            // 64-bit key words (1 or 2)
     int wpow[kMaxD];   // w^l, l < od
     int qblocks;       // workgroups of the tile-queue kernels (0 = automatic)
     int qthreads;      // threads per workgroup of those kernels (64..256, multiple of 64)
     int gw;            // update-tile groups: cells per block of the last coordinate (<= 1: one cell)
 };
+
+// Sub-cells: every cell is split in two along each dimension at its centre
+// ((g + 0.5) * cs, FP32); the sort key carries the d split bits below the
+// cell fields, so the points of a sub-cell are contiguous inside their cell.
+// The last cell of a dimension ends at max((w) * cs, 1) (the clamp of
+// cell_coord).
+__host__ __device__ __forceinline__ float cell_mid(int g, float cs) {
+    return ((float)g + 0.5f) * cs;
+}
 
 // Device-resident loop state (one per solve).
 struct DevState {
@@ -75,6 +86,8 @@ struct DevState {
     int xtile_next;    // work-queue head of the extra check
     int runs;          // runs of equal positions in the sorted order
     int G;             // cell groups of the update tiles
+    int S;             // non-empty sub-cells
+    int WS;            // sub-cell summary work items
     double rc;         // order parameter of the last r_c evaluation
 };
 

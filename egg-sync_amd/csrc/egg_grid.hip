@@ -58,8 +58,9 @@ void launch_fill(int *p, int count, int v, const DevState *st, hipStream_t s) {
     if (count > 0) k_fill<<<grid_for(count), kBlock, 0, s>>>(p, count, v, st);
 }
 
-// Key bit fields: [outer | g_od | ... | g_{d-1}], g_{d-1} least significant,
-// so numeric key order = lexicographic (outer, g_od, ..., g_{d-1}).
+// Key bit fields: [outer | g_od | ... | g_{d-1} | sub_0 ... sub_{d-1}], least
+// significant last, so numeric key order = lexicographic (outer, g_od, ...,
+// g_{d-1}, sub_0, ..., sub_{d-1}); the split bits are present when sub_bits > 0.
 __device__ __forceinline__ void put_bits(uint64_t &lo, uint64_t &hi, uint64_t v, int shift,
                                          int width) {
     if (shift >= 64) {
@@ -77,10 +78,22 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float *__restrict__ X, Ge
     const int n = g.n;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int c[D];
+        float x[D];
 #pragma unroll
-        for (int l = 0; l < D; l++) c[l] = cell_coord(X[(long long)l * n + i], g.cs, g.w);
+        for (int l = 0; l < D; l++) {
+            x[l] = X[(long long)l * n + i];
+            c[l] = cell_coord(x[l], g.cs, g.w);
+        }
         uint64_t lo = 0, hi = 0;
         int shift = 0;
+        if (g.sub_bits) {
+            // split bits, sub_{d-1} least significant
+#pragma unroll
+            for (int l = D - 1; l >= 0; l--) {
+                put_bits(lo, hi, x[l] >= cell_mid(c[l], g.cs) ? 1u : 0u, shift, 1);
+                shift += 1;
+            }
+        }
 #pragma unroll
         for (int l = D - 1; l >= 0; l--) {
             if (l >= g.od) {
@@ -102,20 +115,57 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_gather(
     const float *__restrict__ X, const int *__restrict__ perm, const uint64_t *__restrict__ skeys,
     const int *__restrict__ sidx, Geo g, float *__restrict__ Xs, int *__restrict__ perm_s,
-    int *__restrict__ head, DevState *st_loop) {
+    int *__restrict__ head, int *__restrict__ shead, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
-    const int n = g.n;
+    const int n = g.n, sb = g.sub_bits;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int j = sidx ? sidx[i] : i;
 #pragma unroll
         for (int l = 0; l < D; l++) Xs[(long long)l * n + i] = X[(long long)l * n + j];
         perm_s[i] = perm[j];
-        int h = 1;
+        int h = 1, hs = 1;
         if (i > 0) {
-            h = skeys[i] != skeys[i - 1];
-            if (g.kw == 2) h |= skeys[(long long)n + i] != skeys[(long long)n + i - 1];
+            const uint64_t a0 = skeys[i], b0 = skeys[i - 1];
+            const uint64_t a1 = g.kw == 2 ? skeys[(long long)n + i] : 0;
+            const uint64_t b1 = g.kw == 2 ? skeys[(long long)n + i - 1] : 0;
+            hs = a0 != b0 || a1 != b1;
+            // the cell part: the key without its low sub_bits bits
+            const uint64_t ca0 = sb ? (a0 >> sb) | (a1 << (64 - sb)) : a0;
+            const uint64_t cb0 = sb ? (b0 >> sb) | (b1 << (64 - sb)) : b0;
+            h = ca0 != cb0 || (a1 >> sb) != (b1 >> sb);
         }
         head[i] = h;
+        if (shead) shead[i] = hs;
+    }
+}
+
+// Sub-cell table: first sorted position, fine coordinates 2g + split bit,
+// first sub-cell of each cell.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_subcells(
+    const float *__restrict__ Xs, const int *__restrict__ head, const int *__restrict__ shead,
+    const int *__restrict__ spos, const int *__restrict__ point_cell, Geo g,
+    int *__restrict__ sub_start, int *__restrict__ sub_g, int *__restrict__ cell_sub,
+    DevState *st, DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int n = g.n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int s = spos[i] + shead[i] - 1;
+        if (shead[i]) {
+            sub_start[s] = i;
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                const float x = Xs[(long long)l * n + i];
+                const int gl = cell_coord(x, g.cs, g.w);
+                sub_g[(long long)l * n + s] = 2 * gl + (x >= cell_mid(gl, g.cs) ? 1 : 0);
+            }
+            if (head[i]) cell_sub[point_cell[i]] = s;
+        }
+        if (i == n - 1) {
+            sub_start[s + 1] = n;
+            cell_sub[point_cell[i] + 1] = s + 1;
+            st->S = s + 1;
+        }
     }
 }
 
@@ -176,14 +226,15 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
         KEYS_CASE(5) KEYS_CASE(6) KEYS_CASE(7) KEYS_CASE(8)
     }
 #undef KEYS_CASE
-    int where = radix_sort(b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, g.kw, g.key_bits, b.hist,
+    int where = radix_sort(b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, g.kw, g.sort_bits, b.hist,
                            b.scan_tmp, st_loop, s);
     const uint64_t *skeys = where == 1 ? b.keys_b : b.keys_a;
     const int *sidx = where == 1 ? b.vals_b : (where == 0 ? b.vals_a : nullptr);
 #define GATHER_CASE(DD)                                                                  \
     case DD:                                                                             \
         k_gather<DD><<<gb, kBlock, 0, s>>>(b.X, b.perm, skeys, sidx, g, b.Xs, b.perm_s,  \
-                                           b.head, st_loop);                            \
+                                           b.head, g.sub_bits ? b.shead : nullptr,      \
+                                           st_loop);                                    \
         break;
     switch (g.d) {
         GATHER_CASE(1) GATHER_CASE(2) GATHER_CASE(3) GATHER_CASE(4)
@@ -204,16 +255,32 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
     }
 #undef CELLS_CASE
     k_outer_ranges<<<gb, kBlock, 0, s>>>(b.cell_outer, st, b.ob, b.oe, st_loop);
+    if (g.sub_bits) {
+        scan_exclusive(b.shead, b.spos, n, nullptr, b.scan_tmp, nullptr, st_loop, s);
+#define SUB_CASE(DD)                                                                          \
+    case DD:                                                                                  \
+        k_subcells<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.head, b.shead, b.spos, b.point_cell, g,  \
+                                             b.sub_start, b.sub_g, b.cell_sub, st, st_loop);  \
+        break;
+        switch (g.d) {
+            SUB_CASE(1) SUB_CASE(2) SUB_CASE(3) SUB_CASE(4)
+            SUB_CASE(5) SUB_CASE(6) SUB_CASE(7) SUB_CASE(8)
+        }
+#undef SUB_CASE
+    }
 }
 
 // ------------------------------------------------------------- summaries --
 
+// Summaries work on "groups" of consecutive sorted points -- cells, or
+// sub-cells -- given by their start offsets, coordinates and a count in
+// device memory (*count groups, *nwork work items).
 __global__ __launch_bounds__(kBlock) void k_wi_count(const int *__restrict__ cell_start,
-                                                     const DevState *st,
+                                                     const int *count,
                                                      int *__restrict__ nchunk,
                                                      DevState *st_loop) {
     if (st_loop && st_loop->done) return;
-    const int C = st->C;
+    const int C = *count;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
         const int cnt = cell_start[c + 1] - cell_start[c];
         nchunk[c] = (cnt + kSumChunk - 1) / kSumChunk;
@@ -324,10 +391,11 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_summary(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const int *__restrict__ wi_off,
-    const int *__restrict__ nchunk, Geo g, const DevState *st, float *__restrict__ Ts,
-    float *__restrict__ Tc, float *__restrict__ partial, DevState *st_loop) {
+    const int *__restrict__ nchunk, Geo g, const int *count, const int *nwork,
+    float *__restrict__ Ts, float *__restrict__ Tc, float *__restrict__ partial,
+    DevState *st_loop) {
     if (st_loop && st_loop->done) return;
-    const int n = g.n, C = st->C, W = st->W;
+    const int n = g.n, C = *count, W = *nwork;
     const int lane = __lane_id();
     const int waves = gridDim.x * (blockDim.x / kWave);
     for (int wi = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; wi < W; wi += waves) {
@@ -397,11 +465,11 @@ __global__ __launch_bounds__(kBlock) void k_summary(
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_summary_combine(
     const int *__restrict__ cell_start, const int *__restrict__ cell_g,
-    const int *__restrict__ wi_off, const int *__restrict__ nchunk, Geo g, const DevState *st,
+    const int *__restrict__ wi_off, const int *__restrict__ nchunk, Geo g, const int *count,
     float *__restrict__ Ts, float *__restrict__ Tc, const float *__restrict__ partial,
     DevState *st_loop) {
     if (st_loop && st_loop->done) return;
-    const int n = g.n, C = st->C;
+    const int n = g.n, C = *count;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
         const int nch = nchunk[c];
         if (nch <= 1) continue;
@@ -432,28 +500,52 @@ __global__ void k_store_total(const int *__restrict__ off, const int *__restrict
     *out = m > 0 ? off[m - 1] + cnt[m - 1] : 0;
 }
 
-void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
+// First run of every sub-cell (a sub-cell starts where the position changes,
+// so its first sorted position heads a run; hpos holds the run-head scan).
+__global__ __launch_bounds__(kBlock) void k_sub_runs(const int *__restrict__ sub_start,
+                                                     const int *__restrict__ hpos,
+                                                     const DevState *st,
+                                                     int *__restrict__ sub_run,
+                                                     DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int S = st->S;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x)
+        sub_run[k] = hpos[sub_start[k]];
+    if (blockIdx.x == 0 && threadIdx.x == 0) sub_run[S] = st->runs;
+}
+
+// Kahan per-group totals (see k_summary) of the groups [starts[k], starts[k+1]),
+// k < *count, with anchors coords * g.cs.
+static void group_summaries(const Geo &g, GridBufs &b, const int *starts, const int *coords,
+                            int *nchunk, int *wi_off, const int *count, int *nwork, float *Ts,
+                            float *Tc, DevState *st_loop, hipStream_t s) {
     const int n = g.n;
     const int gb = grid_for(n);
-    k_wi_count<<<gb, kBlock, 0, s>>>(b.cell_start, st, b.nchunk, st_loop);
-    scan_exclusive(b.nchunk, b.wi_off, n, &st->C, b.scan_tmp, nullptr, st_loop, s);
-    k_store_total<<<1, 1, 0, s>>>(b.wi_off, b.nchunk, &st->C, &st->W, st_loop);
-    // work items <= C + n / kSumChunk <= n + n / kSumChunk
+    k_wi_count<<<gb, kBlock, 0, s>>>(starts, count, nchunk, st_loop);
+    scan_exclusive(nchunk, wi_off, n, count, b.scan_tmp, nullptr, st_loop, s);
+    k_store_total<<<1, 1, 0, s>>>(wi_off, nchunk, count, nwork, st_loop);
+    // work items <= groups + n / kSumChunk <= n + n / kSumChunk
     const long long max_wi = (long long)n + n / kSumChunk + 1;
     const int wb = grid_for(max_wi * kWave, kBlock, 16384);
-#define SUM_CASE(DD)                                                                         \
-    case DD:                                                                                 \
-        k_summary<DD><<<wb, kBlock, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.wi_off,          \
-                                            b.nchunk, g, st, b.Ts, b.Tc, b.partial, st_loop); \
-        k_summary_combine<DD><<<gb, kBlock, 0, s>>>(b.cell_start, b.cell_g, b.wi_off,        \
-                                                    b.nchunk, g, st, b.Ts, b.Tc, b.partial,  \
-                                                    st_loop);                                \
+#define SUM_CASE(DD)                                                                          \
+    case DD:                                                                                  \
+        k_summary<DD><<<wb, kBlock, 0, s>>>(b.Xs, starts, coords, wi_off, nchunk, g, count,   \
+                                            nwork, Ts, Tc, b.partial, st_loop);               \
+        k_summary_combine<DD><<<gb, kBlock, 0, s>>>(starts, coords, wi_off, nchunk, g, count, \
+                                                    Ts, Tc, b.partial, st_loop);              \
         break;
     switch (g.d) {
         SUM_CASE(1) SUM_CASE(2) SUM_CASE(3) SUM_CASE(4)
         SUM_CASE(5) SUM_CASE(6) SUM_CASE(7) SUM_CASE(8)
     }
 #undef SUM_CASE
+}
+
+void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
+    const int n = g.n;
+    const int gb = grid_for(n);
+    group_summaries(g, b, b.cell_start, b.cell_g, b.nchunk, b.wi_off, &st->C, &st->W, b.Ts,
+                    b.Tc, st_loop, s);
     // runs of equal positions and the update tiles over them
 #define RUN_CASE(DD) \
     case DD: k_run_heads<DD><<<gb, kBlock, 0, s>>>(b.Xs, n, b.head, st_loop); break;
@@ -471,6 +563,14 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     k_tile_count<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.ntile, st_loop);
     scan_exclusive(b.ntile, b.tile_off, n, &st->G, b.scan_tmp, nullptr, st_loop, s);
     k_store_total<<<1, 1, 0, s>>>(b.tile_off, b.ntile, &st->G, &st->T, st_loop);
+    if (g.sub_bits) {
+        // sub-cell runs and totals (anchors on the half-side grid)
+        k_sub_runs<<<gb, kBlock, 0, s>>>(b.sub_start, b.hpos, st, b.sub_run, st_loop);
+        Geo gf = g;
+        gf.cs = g.cs * 0.5f;
+        group_summaries(gf, b, b.sub_start, b.sub_g, b.snchunk, b.swi_off, &st->S, &st->WS,
+                        b.TsS, b.TcS, st_loop, s);
+    }
 }
 
 }  // namespace egg

@@ -41,6 +41,16 @@ struct GridBufs {
     // order); head / hpos are reused for the run-head flags and their scan
     int *run_start;      // [n+1] first sorted position of each run
     int *cell_run;       // [n+1] first run of each cell
+    // sub-cells (see cell_mid): shead / spos head flags and their scan,
+    // first sorted position, first sub-cell of each cell, fine coordinates
+    // 2g + split bit, first run, totals as Ts / Tc, summary work items
+    int *shead, *spos;   // [n]
+    int *sub_start;      // [n+1]
+    int *cell_sub;       // [n+1]
+    int *sub_g;          // [d][n]
+    int *sub_run;        // [n+1]
+    float *TsS, *TcS;    // [d][n]
+    int *snchunk, *swi_off; // [n]
     // update tiles: cells are grouped (consecutive cells of one outer cell,
     // equal inner coordinates but the last, last one in the same block of gw
     // cells); a tile holds <= 64 runs of one group

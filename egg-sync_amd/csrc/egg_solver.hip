@@ -113,7 +113,9 @@ int compute_geo(int64_t n, int d, float eps, Geo &g) {
     g.bits_o = bitlen(O - 1);
     g.key_bits = g.bits_o + (d - od) * g.bits_w;
     if (g.key_bits > 128) return EGG_SYNC_E_KEY;
-    g.kw = g.key_bits > 64 ? 2 : 1;
+    g.sub_bits = g.key_bits + d <= 128 ? d : 0;
+    g.sort_bits = g.key_bits + g.sub_bits;
+    g.kw = g.sort_bits > 64 ? 2 : 1;
     return EGG_SYNC_OK;
 }
 
@@ -170,6 +172,16 @@ Layout carve(void *base, const Geo &g) {
     b.wi_off = c.take<int>(n);
     b.run_start = c.take<int>(n + 1);
     b.cell_run = c.take<int>(n + 1);
+    b.shead = c.take<int>(n);
+    b.spos = c.take<int>(n);
+    b.sub_start = c.take<int>(n + 1);
+    b.cell_sub = c.take<int>(n + 1);
+    b.sub_g = c.take<int>(n * d);
+    b.sub_run = c.take<int>(n + 1);
+    b.TsS = c.take<float>(n * d);
+    b.TcS = c.take<float>(n * d);
+    b.snchunk = c.take<int>(n);
+    b.swi_off = c.take<int>(n);
     b.gflag = c.take<int>(n);
     b.gpos = c.take<int>(n);
     b.grp_cell = c.take<int>(n + 1);

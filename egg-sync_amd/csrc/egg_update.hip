@@ -241,6 +241,37 @@ __device__ __forceinline__ int next_tile(int *counter) {
     return __builtin_amdgcn_readfirstlane(t);
 }
 
+// Classifies sub-cell sc for a lane at x: 1 if every point of it is within
+// eps (its FP32 squared distance <= thr), 0 if none is, 2 otherwise.  Exact
+// FP64 distances to the sub-cell box, padded by kSubPad against the rounding
+// of cell boundaries, compared with bounds that absorb the FP32 rounding of
+// the reference's distance.
+constexpr double kSubPad = 1e-6;
+
+template <int D>
+__device__ __forceinline__ int sub_class(const int *__restrict__ sub_g, int n, int sc,
+                                         const float *x, const Geo &g, double thr_in,
+                                         double thr_out) {
+    double mn = 0.0, mx = 0.0;
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        const int f = sub_g[(long long)l * n + sc];
+        const int gl = f >> 1;
+        const float top = gl == g.w - 1 ? fmaxf((float)(gl + 1) * g.cs, 1.0f)
+                                        : (float)(gl + 1) * g.cs;
+        const float lo = (f & 1) ? cell_mid(gl, g.cs) : (float)gl * g.cs;
+        const float hi = (f & 1) ? top : cell_mid(gl, g.cs);
+        const double dlo = (double)lo - kSubPad, dhi = (double)hi + kSubPad;
+        const double xd = (double)x[l];
+        const double a = (dlo - xd) * (dlo - xd), b = (dhi - xd) * (dhi - xd);
+        mx += a > b ? a : b;
+        mn += xd < dlo ? a : (xd > dhi ? b : 0.0);
+    }
+    if (mx <= thr_in) return 1;
+    if (mn >= thr_out) return 0;
+    return 2;
+}
+
 // kernel_itr_grid_STAD5_5_1 (EGG_SynC.cu:551-763).  Per lane: full cells add
 // the cell totals (EGG_SynC.cu:648-663); partial cells are streamed through
 // LDS in chunks of 64 runs -- each lane loads one run and computes sin v and
@@ -260,6 +291,9 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ run_start,
     const int *__restrict__ cell_run, const int *__restrict__ grp_cell,
     const int *__restrict__ point_cell, const int *__restrict__ tile_off,
+    const int *__restrict__ cell_sub, const int *__restrict__ sub_start,
+    const int *__restrict__ sub_g, const int *__restrict__ sub_run,
+    const float *__restrict__ TsS, const float *__restrict__ TcS,
     float *__restrict__ rres, int *__restrict__ rdbg, Geo g, DevState *st) {
     if (st->done) return;
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
@@ -267,6 +301,11 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     const int n = g.n, R = g.R, od = g.od;
     const int G = st->G, T = st->T;
     const float thr = g.thr_eps, thr_half = g.thr_half;
+    // sub-cell classification bounds: the reference's FP32 squared distance
+    // is within (d+1) * 2^-24 relative of the exact one, so an exact bound
+    // below thr_in (above thr_out) decides every point of the sub-cell
+    const double thr_in = (double)thr * (1.0 - 64.0 * 0x1p-24);
+    const double thr_out = (double)thr * (1.0 + 64.0 * 0x1p-24);
     const int lane = __lane_id();
     SrcPt<D> *src = s_src[threadIdx.x / kWave];
     int *len = s_len[threadIdx.x / kWave];
@@ -303,6 +342,112 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         }
         const int g0od = select_dim<D>(g0, od);
         int N = 0, half = 0, m_part = 0;
+        // Streams runs [r0, r1) through LDS, 64 per chunk: lanes with `acc`
+        // test each run against eps and accumulate it (counted with its
+        // length; a run's terms are added once per point, in order, as a
+        // pointwise loop would), lanes with `hcnt` count it if within eps/2.
+        auto scan_runs = [&](int r0, int r1, bool acc, bool hcnt) {
+            const bool any_acc = __any(acc);
+            for (int base = r0; base < r1; base += kWave) {
+                const int rr = base + lane;
+                int my_len = 1;
+                if (rr < r1) {
+                    const int j = run_start[rr];
+                    my_len = run_start[rr + 1] - j;
+                    len[lane] = my_len;
+#pragma unroll
+                    for (int l = 0; l < D; l++) {
+                        const float xq = Xs[(long long)l * n + j];
+                        src[lane].x[l] = xq;
+                        if (any_acc) sincosm1_small(xq - a0[l], src[lane].sv[l], src[lane].cv[l]);
+                    }
+                }
+                const bool multi = __any(my_len > 1);
+                wave_sync();
+                const int m = min(kWave, r1 - base);
+                if (acc && !multi) {
+                    // single-point runs: four sources per step so their
+                    // LDS reads overlap
+                    int k = 0;
+                    for (; k + 4 <= m; k += 4) {
+                        float sq[4];
+#pragma unroll
+                        for (int jj = 0; jj < 4; jj++) {
+                            float q[D];
+#pragma unroll
+                            for (int l = 0; l < D; l++) q[l] = src[k + jj].x[l];
+                            sq[jj] = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float df = x[l] - q[l];
+                                sq[jj] += df * df;
+                            }
+                        }
+#pragma unroll
+                        for (int jj = 0; jj < 4; jj++) {
+                            if (sq[jj] <= thr) {
+#pragma unroll
+                                for (int l = 0; l < D; l++) {
+                                    ssv[l] += src[k + jj].sv[l];
+                                    scm[l] += src[k + jj].cv[l];
+                                }
+                                m_part += 1;
+                            }
+                        }
+                    }
+                    for (; k < m; k++) {
+                        const SrcPt<D> &q = src[k];
+                        float sq = 0.f;
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            const float df = x[l] - q.x[l];
+                            sq += df * df;
+                        }
+                        if (sq <= thr) {
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                ssv[l] += q.sv[l];
+                                scm[l] += q.cv[l];
+                            }
+                            m_part += 1;
+                        }
+                    }
+                } else if (acc) {
+                    for (int k = 0; k < m; k++) {
+                        const SrcPt<D> &q = src[k];
+                        float sq = 0.f;
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            const float df = x[l] - q.x[l];
+                            sq += df * df;
+                        }
+                        if (sq <= thr) {
+                            const int lk = len[k];
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float a = q.sv[l], b = q.cv[l];
+                                for (int e = 0; e < lk; e++) {
+                                    ssv[l] += a;
+                                    scm[l] += b;
+                                }
+                            }
+                            m_part += lk;
+                        }
+                    }
+                } else if (hcnt) {
+                    for (int k = 0; k < m; k++) {
+                        float sq = 0.f;
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            const float df = x[l] - src[k].x[l];
+                            sq += df * df;
+                        }
+                        if (sq <= thr_half) half += len[k];
+                    }
+                }
+                wave_sync();
+            }
+        };
         walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
             if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
             for (int c = cb; c < ce; c++) {
@@ -329,113 +474,32 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                     if (border && c == cl) half += cnt;
                 }
                 const bool need_half = full && border && c != cl;
-                const bool any_part = __any(part);
-                if (!any_part && !__any(need_half)) continue;
-                // the cell's runs, 64 per chunk: one test per run, each
-                // counted with its length; a run's term is added once per
-                // point, in order, as a pointwise loop would
-                const int r0 = cell_run[c], r1 = cell_run[c + 1];
-                for (int base = r0; base < r1; base += kWave) {
-                    const int rr = base + lane;
-                    int my_len = 1;
-                    if (rr < r1) {
-                        const int j = run_start[rr];
-                        my_len = run_start[rr + 1] - j;
-                        len[lane] = my_len;
+                const bool any_part = __any(part), any_half = __any(need_half);
+                if (!any_part && !any_half) continue;
+                if (any_half) scan_runs(cell_run[c], cell_run[c + 1], false, need_half);
+                if (!any_part) continue;
+                if (!g.sub_bits) {
+                    scan_runs(cell_run[c], cell_run[c + 1], part, false);
+                    continue;
+                }
+                // sub-cells of a partial cell: those wholly within eps of the
+                // lane add their totals, those wholly beyond are skipped, the
+                // rest are tested point by point
+                for (int sc = cell_sub[c]; sc < cell_sub[c + 1]; sc++) {
+                    int cls = 0;
+                    if (part) cls = sub_class<D>(sub_g, n, sc, x, g, thr_in, thr_out);
+                    if (cls == 1) {
 #pragma unroll
                         for (int l = 0; l < D; l++) {
-                            const float xq = Xs[(long long)l * n + j];
-                            src[lane].x[l] = xq;
-                            if (any_part) {
-                                const float v = xq - a0[l];
-                                sincosm1_small(v, src[lane].sv[l], src[lane].cv[l]);
-                            }
+                            const float a = TsS[(long long)l * n + sc] * cx[l];
+                            const float b = TcS[(long long)l * n + sc] * sx[l];
+                            const float tt = a - b;
+                            sum[l] += tt;
                         }
+                        N += sub_start[sc + 1] - sub_start[sc];
                     }
-                    const bool multi = __any(my_len > 1);
-                    wave_sync();
-                    const int m = min(kWave, r1 - base);
-                    if (part && !multi) {
-                        // every run of the chunk is a single point; four
-                        // sources per step so their LDS reads overlap
-                        int k = 0;
-                        for (; k + 4 <= m; k += 4) {
-                            float s[4];
-#pragma unroll
-                            for (int j = 0; j < 4; j++) {
-                                float q[D];
-#pragma unroll
-                                for (int l = 0; l < D; l++) q[l] = src[k + j].x[l];
-                                s[j] = 0.f;
-#pragma unroll
-                                for (int l = 0; l < D; l++) {
-                                    const float df = x[l] - q[l];
-                                    s[j] += df * df;
-                                }
-                            }
-#pragma unroll
-                            for (int j = 0; j < 4; j++) {
-                                if (s[j] <= thr) {
-#pragma unroll
-                                    for (int l = 0; l < D; l++) {
-                                        ssv[l] += src[k + j].sv[l];
-                                        scm[l] += src[k + j].cv[l];
-                                    }
-                                    m_part += 1;
-                                }
-                            }
-                        }
-                        for (; k < m; k++) {
-                            const SrcPt<D> &q = src[k];
-                            float s = 0.f;
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                const float df = x[l] - q.x[l];
-                                s += df * df;
-                            }
-                            if (s <= thr) {
-#pragma unroll
-                                for (int l = 0; l < D; l++) {
-                                    ssv[l] += q.sv[l];
-                                    scm[l] += q.cv[l];
-                                }
-                                m_part += 1;
-                            }
-                        }
-                    } else if (part) {
-                        for (int k = 0; k < m; k++) {
-                            const SrcPt<D> &q = src[k];
-                            float s = 0.f;
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                const float df = x[l] - q.x[l];
-                                s += df * df;
-                            }
-                            if (s <= thr) {
-                                const int lk = len[k];
-#pragma unroll
-                                for (int l = 0; l < D; l++) {
-                                    const float a = q.sv[l], b = q.cv[l];
-                                    for (int e = 0; e < lk; e++) {
-                                        ssv[l] += a;
-                                        scm[l] += b;
-                                    }
-                                }
-                                m_part += lk;
-                            }
-                        }
-                    } else if (need_half) {
-                        for (int k = 0; k < m; k++) {
-                            float s = 0.f;
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                const float df = x[l] - src[k].x[l];
-                                s += df * df;
-                            }
-                            if (s <= thr_half) half += len[k];
-                        }
-                    }
-                    wave_sync();
+                    const bool straddle = cls == 2;
+                    if (__any(straddle)) scan_runs(sub_run[sc], sub_run[sc + 1], straddle, false);
                 }
             }
         });
@@ -882,8 +946,9 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
     case DD:                                                                                  \
         k_update_tile<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc,     \
                                             b.ob, b.oe, b.run_start, b.cell_run,              \
-                                            b.grp_cell, b.point_cell, b.tile_off, b.rres,     \
-                                            rdbg, g, st);                                     \
+                                            b.grp_cell, b.point_cell, b.tile_off,             \
+                                            b.cell_sub, b.sub_start, b.sub_g, b.sub_run,      \
+                                            b.TsS, b.TcS, b.rres, rdbg, g, st);               \
         k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.head, b.hpos, b.rres, rdbg,     \
                                                             b.perm_s, b.X, b.perm, g.n, st,   \
                                                             d);                               \

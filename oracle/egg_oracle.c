@@ -108,11 +108,25 @@ static inline int border_f32(float x, float cs) {
 #undef LE_HALF
 #undef LE_EXTRA
 
+static int bitlen64(int64_t v) {
+    int b = 0;
+    while (v > 0) { b++; v >>= 1; }
+    return b < 1 ? 1 : b;
+}
+
+/* Sub-cell ordering is on when the GPU's sort key (cell fields + d split
+ * bits) fits 128 bits. */
+static int sub_cells_on(const eo_geom *g) {
+    int key_bits = bitlen64(g->O - 1) + (g->d - g->od) * bitlen64((int64_t)g->w - 1);
+    return key_bits + g->d <= 128;
+}
+
 static void geo_f32_from(const eo_geom *g, geo_t_f32 *G) {
     G->n = g->n; G->d = g->d; G->w = g->w; G->od = g->od; G->R = g->R;
     G->R_extra = g->R_extra; G->R_half = g->R_half; G->O = g->O;
     G->eps = g->eps; G->cs = g->cs; G->eps_extra = g->eps_extra;
     G->thr_eps = g->thr_eps; G->thr_half = g->thr_half; G->thr_extra = g->thr_extra;
+    G->sub = sub_cells_on(g);
 }
 
 /* ---------------------------------------------------------------- FP64 -- */

@@ -117,12 +117,21 @@ def test_grid_invariants(O, d, n, eps, seed):
         outer += cg[:, l] * g.w ** l
     keys = [tuple([o] + list(r[g.od:])) for o, r in zip(outer, cg)]
     assert keys == sorted(keys) and len(set(keys)) == len(keys)
-    # stability: within a cell, points keep their order_in order
+    # inside a cell, points are ordered by sub-cell (split bits at the cell
+    # centre, computed in FP32, dimension 0 first), and stably within one
+    mid = ((coords.astype(f32) + f32(0.5)) * f32(g.cs)).astype(f32)
+    sub = (X >= mid).astype(np.int64)
+    sub_key = np.zeros(n, np.int64)
+    for l in range(d):
+        sub_key = sub_key * 2 + sub[:, l]
     rank = np.empty(n, np.int64)
     rank[order_in] = np.arange(n)
     for c in range(G["C"]):
-        r = rank[order[cs_[c]:cs_[c + 1]]]
-        assert np.all(np.diff(r) > 0)
+        pts = order[cs_[c]:cs_[c + 1]]
+        sk = sub_key[pts]
+        assert np.all(np.diff(sk) >= 0)
+        for v in np.unique(sk):
+            assert np.all(np.diff(rank[pts[sk == v]]) > 0)
     for o in np.unique(outer):
         cells = np.nonzero(outer == o)[0]
         assert G["ob"][o] == cells[0] and G["oe"][o] == cells[-1] + 1
