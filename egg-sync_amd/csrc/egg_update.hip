@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         // test each run against eps and accumulate it (counted with its
         // length; a run's terms are added once per point, in order, as a
         // pointwise loop would), lanes with `hcnt` count it if within eps/2.
-        auto scan_runs = [&](int r0, int r1, bool acc, bool hcnt) {
+        auto scan_runs = [&](int r0, int r1, bool acc, bool hcnt) __attribute__((always_inline)) {
             const bool any_acc = __any(acc);
             for (int base = r0; base < r1; base += kWave) {
                 const int rr = base + lane;
