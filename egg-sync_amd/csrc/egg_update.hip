@@ -241,6 +241,13 @@ __device__ __forceinline__ int next_tile(int *counter) {
     return __builtin_amdgcn_readfirstlane(t);
 }
 
+// Bitwise OR of v over the wavefront.
+__device__ __forceinline__ unsigned wave_or(unsigned v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v |= (unsigned)__shfl_xor((int)v, o, kWave);
+    return v;
+}
+
 // Classifies sub-cell sc for a lane at x: 1 if every point of it is within
 // eps (its FP32 squared distance <= thr), 0 if none is, 2 otherwise.  Exact
 // FP64 distances to the sub-cell box, padded by kSubPad against the rounding
@@ -342,16 +349,38 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         }
         const int g0od = select_dim<D>(g0, od);
         int N = 0, half = 0, m_part = 0;
-        // Streams runs [r0, r1) through LDS, 64 per chunk: lanes with `acc`
-        // test each run against eps and accumulate it (counted with its
-        // length; a run's terms are added once per point, in order, as a
-        // pointwise loop would), lanes with `hcnt` count it if within eps/2.
-        auto scan_runs = [&](int r0, int r1, bool acc, bool hcnt) __attribute__((always_inline)) {
+        // Streams runs through LDS, 64 per chunk: runs [r0, r1) if smask is
+        // 0, else the runs of the sub-cells s0 + k of the bits k of smask, in
+        // order.  Lanes with `acc` test each run against eps and accumulate
+        // it (counted with its length; a run's terms are added once per
+        // point, in order, as a pointwise loop would).
+        auto scan_runs = [&](int r0, int r1, int s0, unsigned smask,
+                             bool acc) __attribute__((always_inline)) {
             const bool any_acc = __any(acc);
-            for (int base = r0; base < r1; base += kWave) {
-                const int rr = base + lane;
+            int total = r1 - r0;
+            if (smask) {
+                total = 0;
+                for (unsigned mm = smask; mm; mm &= mm - 1) {
+                    const int k = __builtin_ctz(mm);
+                    total += sub_run[s0 + k + 1] - sub_run[s0 + k];
+                }
+            }
+            for (int base = 0; base < total; base += kWave) {
+                const int q = base + lane;
+                int rr = -1;
+                if (!smask) {
+                    if (q < total) rr = r0 + q;
+                } else {
+                    int before = 0;
+                    for (unsigned mm = smask; mm; mm &= mm - 1) {
+                        const int k = __builtin_ctz(mm);
+                        const int a = sub_run[s0 + k], b = sub_run[s0 + k + 1];
+                        if (q >= before && q < before + (b - a)) rr = a + q - before;
+                        before += b - a;
+                    }
+                }
                 int my_len = 1;
-                if (rr < r1) {
+                if (rr >= 0) {
                     const int j = run_start[rr];
                     my_len = run_start[rr + 1] - j;
                     len[lane] = my_len;
@@ -364,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                 }
                 const bool multi = __any(my_len > 1);
                 wave_sync();
-                const int m = min(kWave, r1 - base);
+                const int m = min(kWave, total - base);
                 if (acc && !multi) {
                     // single-point runs: four sources per step so their
                     // LDS reads overlap
@@ -373,13 +402,13 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                         float sq[4];
 #pragma unroll
                         for (int jj = 0; jj < 4; jj++) {
-                            float q[D];
+                            float qx[D];
 #pragma unroll
-                            for (int l = 0; l < D; l++) q[l] = src[k + jj].x[l];
+                            for (int l = 0; l < D; l++) qx[l] = src[k + jj].x[l];
                             sq[jj] = 0.f;
 #pragma unroll
                             for (int l = 0; l < D; l++) {
-                                const float df = x[l] - q[l];
+                                const float df = x[l] - qx[l];
                                 sq[jj] += df * df;
                             }
                         }
@@ -396,36 +425,36 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                         }
                     }
                     for (; k < m; k++) {
-                        const SrcPt<D> &q = src[k];
+                        const SrcPt<D> &qq = src[k];
                         float sq = 0.f;
 #pragma unroll
                         for (int l = 0; l < D; l++) {
-                            const float df = x[l] - q.x[l];
+                            const float df = x[l] - qq.x[l];
                             sq += df * df;
                         }
                         if (sq <= thr) {
 #pragma unroll
                             for (int l = 0; l < D; l++) {
-                                ssv[l] += q.sv[l];
-                                scm[l] += q.cv[l];
+                                ssv[l] += qq.sv[l];
+                                scm[l] += qq.cv[l];
                             }
                             m_part += 1;
                         }
                     }
                 } else if (acc) {
                     for (int k = 0; k < m; k++) {
-                        const SrcPt<D> &q = src[k];
+                        const SrcPt<D> &qq = src[k];
                         float sq = 0.f;
 #pragma unroll
                         for (int l = 0; l < D; l++) {
-                            const float df = x[l] - q.x[l];
+                            const float df = x[l] - qq.x[l];
                             sq += df * df;
                         }
                         if (sq <= thr) {
                             const int lk = len[k];
 #pragma unroll
                             for (int l = 0; l < D; l++) {
-                                const float a = q.sv[l], b = q.cv[l];
+                                const float a = qq.sv[l], b = qq.cv[l];
                                 for (int e = 0; e < lk; e++) {
                                     ssv[l] += a;
                                     scm[l] += b;
@@ -434,7 +463,23 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                             m_part += lk;
                         }
                     }
-                } else if (hcnt) {
+                }
+                wave_sync();
+            }
+        };
+        // Lanes with `hcnt` count the points of runs [r0, r1) within eps/2.
+        auto half_scan = [&](int r0, int r1, bool hcnt) __attribute__((always_inline)) {
+            for (int base = r0; base < r1; base += kWave) {
+                const int rr = base + lane;
+                if (rr < r1) {
+                    const int j = run_start[rr];
+                    len[lane] = run_start[rr + 1] - j;
+#pragma unroll
+                    for (int l = 0; l < D; l++) src[lane].x[l] = Xs[(long long)l * n + j];
+                }
+                wave_sync();
+                const int m = min(kWave, r1 - base);
+                if (hcnt) {
                     for (int k = 0; k < m; k++) {
                         float sq = 0.f;
 #pragma unroll
@@ -476,31 +521,44 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                 const bool need_half = full && border && c != cl;
                 const bool any_part = __any(part), any_half = __any(need_half);
                 if (!any_part && !any_half) continue;
-                if (any_half) scan_runs(cell_run[c], cell_run[c + 1], false, need_half);
+                if (any_half) half_scan(cell_run[c], cell_run[c + 1], need_half);
                 if (!any_part) continue;
-                if (!g.sub_bits) {
-                    scan_runs(cell_run[c], cell_run[c + 1], part, false);
-                    continue;
-                }
-                // sub-cells of a partial cell: those wholly within eps of the
-                // lane add their totals, those wholly beyond are skipped, the
-                // rest are tested point by point
-                for (int sc = cell_sub[c]; sc < cell_sub[c + 1]; sc++) {
-                    int cls = 0;
-                    if (part) cls = sub_class<D>(sub_g, n, sc, x, g, thr_in, thr_out);
-                    if (cls == 1) {
-#pragma unroll
-                        for (int l = 0; l < D; l++) {
-                            const float a = TsS[(long long)l * n + sc] * cx[l];
-                            const float b = TcS[(long long)l * n + sc] * sx[l];
-                            const float tt = a - b;
-                            sum[l] += tt;
+                const int s0 = cell_sub[c], ns = cell_sub[c + 1] - s0;
+                unsigned umask = 0;
+                if (g.sub_bits && ns <= 32) {
+                    // sub-cells of a partial cell: each lane classifies them
+                    // (wholly within eps / wholly beyond / straddling); the
+                    // sub-cells some lane straddles are streamed once for the
+                    // wavefront and tested point by point by every lane with
+                    // a partial cell; other sub-cells wholly within eps add
+                    // their totals
+                    unsigned imask = 0, smask = 0;
+                    if (part) {
+                        for (int k = 0; k < ns; k++) {
+                            const int cls = sub_class<D>(sub_g, n, s0 + k, x, g, thr_in, thr_out);
+                            if (cls == 1) imask |= 1u << k;
+                            else if (cls == 2) smask |= 1u << k;
                         }
-                        N += sub_start[sc + 1] - sub_start[sc];
                     }
-                    const bool straddle = cls == 2;
-                    if (__any(straddle)) scan_runs(sub_run[sc], sub_run[sc + 1], straddle, false);
+                    umask = wave_or(smask);
+                    const unsigned tmask = imask & ~umask;
+                    for (unsigned mm = wave_or(tmask); mm; mm &= mm - 1) {
+                        const int k = __builtin_ctz(mm);
+                        const int sc = s0 + k;
+                        if ((tmask >> k) & 1u) {
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float a = TsS[(long long)l * n + sc] * cx[l];
+                                const float b = TcS[(long long)l * n + sc] * sx[l];
+                                const float tt = a - b;
+                                sum[l] += tt;
+                            }
+                            N += sub_start[sc + 1] - sub_start[sc];
+                        }
+                    }
+                    if (!umask) continue;
                 }
+                scan_runs(cell_run[c], cell_run[c + 1], s0, umask, part);
             }
         });
         N += m_part;
