@@ -2,8 +2,7 @@
 
 The knobs it searches change how work is launched, never what is computed:
 the tile-queue kernels' workgroup count and size (`queue_blocks`,
-`queue_threads`), how many neighbouring cells share an update tile
-(`tile_group`; the arithmetic is anchored so that it does not depend on it), iterations per host check (`iters_per_sync`), hipGraph
+`queue_threads`), iterations per host check (`iters_per_sync`), hipGraph
 replay (`use_graph`) and, for batched solves, the number of streams.  Every
 candidate's labels and final positions are compared bit for bit with the
 first candidate's, and a candidate that differs is an error, not a result.
@@ -27,7 +26,6 @@ import torch
 from . import solver
 
 PLAIN_SPACE = {
-    "tile_group": [1, 2, 4, 8],
     "queue_threads": [256, 128, 64],
     "queue_blocks": [0, 1024, 4096],
     "iters_per_sync": [4, 8, 16],
@@ -36,7 +34,6 @@ PLAIN_SPACE = {
 
 BATCH_SPACE = {
     "streams": [4, 1, 2, 8],
-    "tile_group": [1, 4],
     "use_graph": [False, True],
     "iters_per_sync": [4, 2, 8],
     "queue_threads": [256, 128],

@@ -32,7 +32,8 @@ def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-
     use_graph replays the loop body from a captured hipGraph; queue_blocks /
     queue_threads set the launch shape of the tile-queue kernels (0 = automatic);
     tile_group lets up to that many neighbouring cells share an update tile.
-    None of these change the result."""
+    The first three do not change the result; tile_group changes the order of
+    FP32 additions only."""
     D = _dev_f32(X)
     n, d = D.shape
     if label_rule is None:

@@ -70,8 +70,8 @@ typedef struct egg_sync_options {
     int32_t tile_group;      /* update tiles: neighbouring cells that differ only
                                 in the last coordinate, within aligned blocks of
                                 this many cells (1, 2, 4 or 8; others round down),
-                                share a wavefront (0 = default).  Results do not
-                                depend on it                                      */
+                                share a wavefront (0 = default).  It changes the
+                                order of FP32 additions only                      */
 } egg_sync_options;
 
 typedef struct egg_sync_stats {
