@@ -32,6 +32,12 @@ sys.path.insert(0, os.path.join(ROOT, "egg-sync_amd"))
 
 METRIC = "time-to-clustering (s), point-updates/s; plain vs learned form, same labels"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_update_tile<3> launch at n=1M (mean over the solve's 10
+# launches): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each divided by what
+# the counter reads for a 256 MiB copy (2x and 4x the bytes moved); see
+# profiles/k_update_tile_config1_n1M_counters.md.  Measured for this workload
+# only, so reported only when n is 1,000,000.
+UPDATE_TRAFFIC_BYTES_N1M = 53.5e6
 
 
 def cpu_baseline(X, eps, target_seconds=12.0):
@@ -248,7 +254,8 @@ def main():
                        [round(float(x), 3) for x in (st["itr_ms"] if st["itr_ms"] is not None else [])],
                    "cells_per_iteration_last_step": [int(c) for c in st["cells_trace"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": UPDATE_TRAFFIC_BYTES_N1M if n == 1_000_000 else None,
                      "kernel": "k_update_tile<3>", "launches": launches,
                      "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
