@@ -29,7 +29,6 @@ constexpr int kSumChunk = 2048;   // max points per summary work item
 constexpr int kTile = 64;         // max runs per update tile (one wavefront)
 constexpr int kDefaultGroup = 1;  // default cells per tile group along the last coordinate
 constexpr int kAnchorBlock = 8;   // group widths are powers of two dividing this
-constexpr int kParts = 4;         // parts of an update tile's candidate walk
 
 // Grid geometry and thresholds, computed on the host (EGG_SynC.cu:1249-1260).
 struct Geo {

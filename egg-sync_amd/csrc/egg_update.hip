@@ -312,8 +312,8 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     if (st->done) return;
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
     __shared__ int s_len[kBlock / kWave][kWave];
-    __shared__ float s_acc[kParts][3 * D][kWave];  // sum, ssv, scm of each part of the walk
-    __shared__ int s_cnt[kParts][3][kWave];        // N, half, m_part
+    __shared__ float s_acc[kBlock / kWave][3 * D][kWave];  // sum, ssv, scm of each wavefront
+    __shared__ int s_cnt[kBlock / kWave][3][kWave];        // N, half, m_part
     __shared__ int s_tile;
     const int wv = threadIdx.x / kWave, W = blockDim.x / kWave;
     const int n = g.n, R = g.R, od = g.od;
@@ -508,118 +508,98 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                 wave_sync();
             }
         };
-        // kParts logical parts of the walk, part p taken by wavefront p mod W;
-        // each part's sums go to LDS and are combined in part order, so the
-        // result does not depend on the workgroup size
-        for (int part = wv; part < kParts; part += W) {
+        walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
+            if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+            for (int c = cb; c < ce; c++) {
+                int gc[D];
+                bool stop;
+                const bool in =
+                    cell_in_group_window<D>(cell_g, n, c, g0, od, R + 1, ghi, gc, &stop);
+                if (stop) break;
+                if (!in) continue;
+                float sm, lg;
+                box_dist<D>(gc, x, g.cs, sm, lg);
+                const bool full = active && lg <= thr;
+                const bool part = active && !full && sm <= thr;
+                if (full) {
 #pragma unroll
-            for (int l = 0; l < D; l++) {
-                sum[l] = 0.f;
-                ssv[l] = 0.f;
-                scm[l] = 0.f;
-            }
-            N = 0;
-            half = 0;
-            m_part = 0;
-            walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
-                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
-                for (int c = cb; c < ce; c++) {
-                    int gc[D];
-                    bool stop;
-                    const bool in =
-                        cell_in_group_window<D>(cell_g, n, c, g0, od, R + 1, ghi, gc, &stop);
-                    if (stop) break;
-                    if (!in) continue;
-                    float sm, lg;
-                    box_dist<D>(gc, x, g.cs, sm, lg);
-                    const bool full = active && lg <= thr;
-                    const bool part = active && !full && sm <= thr;
-                    if (full) {
-#pragma unroll
-                        for (int l = 0; l < D; l++) {
-                            const float a = Ts[(long long)l * n + c] * cx[l];
-                            const float b = Tc[(long long)l * n + c] * sx[l];
-                            const float tt = a - b;
-                            sum[l] += tt;
-                        }
-                        const int cnt = cell_start[c + 1] - cell_start[c];
-                        N += cnt;
-                        if (border && c == cl) half += cnt;
+                    for (int l = 0; l < D; l++) {
+                        const float a = Ts[(long long)l * n + c] * cx[l];
+                        const float b = Tc[(long long)l * n + c] * sx[l];
+                        const float tt = a - b;
+                        sum[l] += tt;
                     }
-                    const bool need_half = full && border && c != cl;
-                    const bool any_part = __any(part), any_half = __any(need_half);
-                    if (!any_part && !any_half) continue;
-                    if (any_half) half_scan(cell_run[c], cell_run[c + 1], need_half);
-                    if (!any_part) continue;
-                    const int s0 = cell_sub[c], ns = cell_sub[c + 1] - s0;
-                    unsigned umask = 0;
-                    if (g.sub_bits && ns <= 32) {
-                        // sub-cells of a partial cell: each lane classifies them
-                        // (wholly within eps / wholly beyond / straddling); the
-                        // sub-cells some lane straddles are streamed once for the
-                        // wavefront and tested point by point by every lane with
-                        // a partial cell; other sub-cells wholly within eps add
-                        // their totals
-                        unsigned imask = 0, smask = 0;
-                        if (part) {
-                            for (int k = 0; k < ns; k++) {
-                                const int cls = sub_class<D>(sub_g, n, s0 + k, x, g, thr_in, thr_out);
-                                if (cls == 1) imask |= 1u << k;
-                                else if (cls == 2) smask |= 1u << k;
-                            }
-                        }
-                        umask = wave_or(smask);
-                        const unsigned tmask = imask & ~umask;
-                        for (unsigned mm = wave_or(tmask); mm; mm &= mm - 1) {
-                            const int k = __builtin_ctz(mm);
-                            const int sc = s0 + k;
-                            if ((tmask >> k) & 1u) {
-#pragma unroll
-                                for (int l = 0; l < D; l++) {
-                                    const float a = TsS[(long long)l * n + sc] * cx[l];
-                                    const float b = TcS[(long long)l * n + sc] * sx[l];
-                                    const float tt = a - b;
-                                    sum[l] += tt;
-                                }
-                                N += sub_start[sc + 1] - sub_start[sc];
-                            }
-                        }
-                        if (!umask) continue;
-                    }
-                    scan_runs(cell_run[c], cell_run[c + 1], s0, umask, part);
+                    const int cnt = cell_start[c + 1] - cell_start[c];
+                    N += cnt;
+                    if (border && c == cl) half += cnt;
                 }
-            }, part, kParts);
+                const bool need_half = full && border && c != cl;
+                const bool any_part = __any(part), any_half = __any(need_half);
+                if (!any_part && !any_half) continue;
+                if (any_half) half_scan(cell_run[c], cell_run[c + 1], need_half);
+                if (!any_part) continue;
+                const int s0 = cell_sub[c], ns = cell_sub[c + 1] - s0;
+                unsigned umask = 0;
+                if (g.sub_bits && ns <= 32) {
+                    // sub-cells of a partial cell: each lane classifies them
+                    // (wholly within eps / wholly beyond / straddling); the
+                    // sub-cells some lane straddles are streamed once for the
+                    // wavefront and tested point by point by every lane with
+                    // a partial cell; other sub-cells wholly within eps add
+                    // their totals
+                    unsigned imask = 0, smask = 0;
+                    if (part) {
+                        for (int k = 0; k < ns; k++) {
+                            const int cls = sub_class<D>(sub_g, n, s0 + k, x, g, thr_in, thr_out);
+                            if (cls == 1) imask |= 1u << k;
+                            else if (cls == 2) smask |= 1u << k;
+                        }
+                    }
+                    umask = wave_or(smask);
+                    const unsigned tmask = imask & ~umask;
+                    for (unsigned mm = wave_or(tmask); mm; mm &= mm - 1) {
+                        const int k = __builtin_ctz(mm);
+                        const int sc = s0 + k;
+                        if ((tmask >> k) & 1u) {
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float a = TsS[(long long)l * n + sc] * cx[l];
+                                const float b = TcS[(long long)l * n + sc] * sx[l];
+                                const float tt = a - b;
+                                sum[l] += tt;
+                            }
+                            N += sub_start[sc + 1] - sub_start[sc];
+                        }
+                    }
+                    if (!umask) continue;
+                }
+                scan_runs(cell_run[c], cell_run[c + 1], s0, umask, part);
+            }
+        }, wv, W);
+        // the wavefronts' partial sums, combined in wavefront order
+        if (wv > 0) {
 #pragma unroll
             for (int l = 0; l < D; l++) {
-                s_acc[part][l][lane] = sum[l];
-                s_acc[part][D + l][lane] = ssv[l];
-                s_acc[part][2 * D + l][lane] = scm[l];
+                s_acc[wv][l][lane] = sum[l];
+                s_acc[wv][D + l][lane] = ssv[l];
+                s_acc[wv][2 * D + l][lane] = scm[l];
             }
-            s_cnt[part][0][lane] = N;
-            s_cnt[part][1][lane] = half;
-            s_cnt[part][2][lane] = m_part;
+            s_cnt[wv][0][lane] = N;
+            s_cnt[wv][1][lane] = half;
+            s_cnt[wv][2][lane] = m_part;
         }
         __syncthreads();
         if (wv > 0) continue;
-#pragma unroll
-        for (int l = 0; l < D; l++) {
-            sum[l] = s_acc[0][l][lane];
-            ssv[l] = s_acc[0][D + l][lane];
-            scm[l] = s_acc[0][2 * D + l][lane];
-        }
-        N = s_cnt[0][0][lane];
-        half = s_cnt[0][1][lane];
-        m_part = s_cnt[0][2][lane];
-        for (int p = 1; p < kParts; p++) {
+        for (int w = 1; w < W; w++) {
 #pragma unroll
             for (int l = 0; l < D; l++) {
-                sum[l] += s_acc[p][l][lane];
-                ssv[l] += s_acc[p][D + l][lane];
-                scm[l] += s_acc[p][2 * D + l][lane];
+                sum[l] += s_acc[w][l][lane];
+                ssv[l] += s_acc[w][D + l][lane];
+                scm[l] += s_acc[w][2 * D + l][lane];
             }
-            N += s_cnt[p][0][lane];
-            half += s_cnt[p][1][lane];
-            m_part += s_cnt[p][2][lane];
+            N += s_cnt[w][0][lane];
+            half += s_cnt[w][1][lane];
+            m_part += s_cnt[w][2][lane];
         }
         N += m_part;
         float xo[D];
