@@ -67,16 +67,11 @@ struct Odo {
 // adjacent ids, so lane k loads the cell range of offset k and the wavefront
 // visits the non-empty ones in lane order.  f(cb, ce) receives each cell
 // range.  Called by a whole wavefront with wave-uniform g0 and R (2R+1 <= 64).
-// With nparts > 1 the wavefront visits only the outer cells whose running
-// index (row * (2R+1) + column) is part modulo nparts, so nparts wavefronts
-// share one walk.
 template <int D, typename Fn>
 __device__ __forceinline__ void walk_columns(const int *g0, int R, const Geo &g,
                                              const int *__restrict__ ob,
-                                             const int *__restrict__ oe, Fn &&f, int part = 0,
-                                             int nparts = 1) {
+                                             const int *__restrict__ oe, Fn &&f) {
     const int od = g.od, lane = __lane_id();
-    int row = 0;
     int idx[D];
 #pragma unroll
     for (int l = 0; l < D; l++) idx[l] = -R;
@@ -98,15 +93,13 @@ __device__ __forceinline__ void walk_columns(const int *g0, int R, const Geo &g,
                 cb = ob[base + v0];
                 ce = oe[base + v0];
             }
-            const bool mine = nparts == 1 || (row * (2 * R + 1) + lane) % nparts == part;
-            unsigned long long m = __ballot(ce > cb && mine);
+            unsigned long long m = __ballot(ce > cb);
             while (m) {
                 const int k = __ffsll((long long)m) - 1;
                 m &= m - 1;
                 f(__builtin_amdgcn_readlane(cb, k), __builtin_amdgcn_readlane(ce, k));
             }
         }
-        row++;
         bool carry = true;
 #pragma unroll
         for (int l = 1; l < D; l++) {
@@ -312,10 +305,6 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     if (st->done) return;
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
     __shared__ int s_len[kBlock / kWave][kWave];
-    __shared__ float s_acc[kBlock / kWave][3 * D][kWave];  // sum, ssv, scm of each wavefront
-    __shared__ int s_cnt[kBlock / kWave][3][kWave];        // N, half, m_part
-    __shared__ int s_tile;
-    const int wv = threadIdx.x / kWave, W = blockDim.x / kWave;
     const int n = g.n, R = g.R, od = g.od;
     const int G = st->G, T = st->T;
     const float thr = g.thr_eps, thr_half = g.thr_half;
@@ -330,11 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     const int *cg_od = cell_g + (long long)od * n;
     bool unsynced = false;
     for (;;) {
-        // the workgroup takes one tile; its wavefronts share the walk
-        if (threadIdx.x == 0) s_tile = atomicAdd(&st->tile_next, 1);
-        __syncthreads();
-        const int t = s_tile;
-        __syncthreads();
+        const int t = next_tile(&st->tile_next);
         if (t >= T) break;
         const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
         const int c0 = tl.gc0, cl = tl.cl, r = tl.r, i = tl.i;
@@ -575,32 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                 }
                 scan_runs(cell_run[c], cell_run[c + 1], s0, umask, part);
             }
-        }, wv, W);
-        // the wavefronts' partial sums, combined in wavefront order
-        if (wv > 0) {
-#pragma unroll
-            for (int l = 0; l < D; l++) {
-                s_acc[wv][l][lane] = sum[l];
-                s_acc[wv][D + l][lane] = ssv[l];
-                s_acc[wv][2 * D + l][lane] = scm[l];
-            }
-            s_cnt[wv][0][lane] = N;
-            s_cnt[wv][1][lane] = half;
-            s_cnt[wv][2][lane] = m_part;
-        }
-        __syncthreads();
-        if (wv > 0) continue;
-        for (int w = 1; w < W; w++) {
-#pragma unroll
-            for (int l = 0; l < D; l++) {
-                sum[l] += s_acc[w][l][lane];
-                ssv[l] += s_acc[w][D + l][lane];
-                scm[l] += s_acc[w][2 * D + l][lane];
-            }
-            N += s_cnt[w][0][lane];
-            half += s_cnt[w][1][lane];
-            m_part += s_cnt[w][2][lane];
-        }
+        });
         N += m_part;
         float xo[D];
 #pragma unroll
