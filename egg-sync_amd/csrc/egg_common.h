@@ -50,7 +50,8 @@ struct Geo {
     int bits_o;        // bits of the outer id field
     int key_bits;      // bits of the cell part of the key
     int sub_bits;      // bits of the sub-cell part below it (d, or 0 if keys would exceed 128 bits)
-    int sort_bits;     // key_bits + sub_bits
+    int hash_bits;     // bits of the position hash below that (<= 16)
+    int sort_bits;     // key_bits + sub_bits + hash_bits
     int kw;            // 64-bit key words (1 or 2)
     int wpow[kMaxD];   // w^l, l < od
     int qblocks;       // workgroups of the tile-queue kernels (0 = automatic)
@@ -65,6 +66,22 @@ struct Geo {
 // cell_coord).
 __host__ __device__ __forceinline__ float cell_mid(int g, float cs) {
     return ((float)g + 0.5f) * cs;
+}
+
+// Hash of a position's FP32 bit patterns (the lowest field of the sort key):
+// bitwise-equal positions get equal hashes, so inside a sub-cell they sort
+// next to each other and form one run.
+__host__ __device__ __forceinline__ uint32_t pos_hash_step(uint32_t h, uint32_t bits) {
+    h ^= bits;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    return h;
+}
+
+__host__ __device__ __forceinline__ uint32_t pos_hash_final(uint32_t h) {
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
 }
 
 // Device-resident loop state (one per solve).

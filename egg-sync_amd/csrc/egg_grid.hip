@@ -58,9 +58,10 @@ void launch_fill(int *p, int count, int v, const DevState *st, hipStream_t s) {
     if (count > 0) k_fill<<<grid_for(count), kBlock, 0, s>>>(p, count, v, st);
 }
 
-// Key bit fields: [outer | g_od | ... | g_{d-1} | sub_0 ... sub_{d-1}], least
-// significant last, so numeric key order = lexicographic (outer, g_od, ...,
-// g_{d-1}, sub_0, ..., sub_{d-1}); the split bits are present when sub_bits > 0.
+// Key bit fields: [outer | g_od | ... | g_{d-1} | sub_0 ... sub_{d-1} | hash],
+// least significant last, so numeric key order = lexicographic (outer, g_od,
+// ..., g_{d-1}, sub_0, ..., sub_{d-1}, position hash); the split bits and the
+// hash are present when sub_bits / hash_bits > 0.
 __device__ __forceinline__ void put_bits(uint64_t &lo, uint64_t &hi, uint64_t v, int shift,
                                          int width) {
     if (shift >= 64) {
@@ -86,6 +87,14 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float *__restrict__ X, Ge
         }
         uint64_t lo = 0, hi = 0;
         int shift = 0;
+        if (g.hash_bits) {
+            uint32_t h = 0x9E3779B9u;
+#pragma unroll
+            for (int l = 0; l < D; l++) h = pos_hash_step(h, __float_as_uint(x[l]));
+            h = pos_hash_final(h);
+            put_bits(lo, hi, (uint64_t)(h & ((1u << g.hash_bits) - 1u)), 0, g.hash_bits);
+            shift = g.hash_bits;
+        }
         if (g.sub_bits) {
             // split bits, sub_{d-1} least significant
 #pragma unroll
@@ -117,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(
     const int *__restrict__ sidx, Geo g, float *__restrict__ Xs, int *__restrict__ perm_s,
     int *__restrict__ head, int *__restrict__ shead, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
-    const int n = g.n, sb = g.sub_bits;
+    const int n = g.n, sb = g.sub_bits + g.hash_bits, hb = g.hash_bits;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int j = sidx ? sidx[i] : i;
 #pragma unroll
@@ -128,8 +137,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(
             const uint64_t a0 = skeys[i], b0 = skeys[i - 1];
             const uint64_t a1 = g.kw == 2 ? skeys[(long long)n + i] : 0;
             const uint64_t b1 = g.kw == 2 ? skeys[(long long)n + i - 1] : 0;
-            hs = a0 != b0 || a1 != b1;
-            // the cell part: the key without its low sub_bits bits
+            // the sub-cell part: the key without its low hash_bits bits;
+            // the cell part: without the split bits either
+            const uint64_t sa0 = hb ? (a0 >> hb) | (a1 << (64 - hb)) : a0;
+            const uint64_t sb0 = hb ? (b0 >> hb) | (b1 << (64 - hb)) : b0;
+            hs = sa0 != sb0 || (a1 >> hb) != (b1 >> hb);
             const uint64_t ca0 = sb ? (a0 >> sb) | (a1 << (64 - sb)) : a0;
             const uint64_t cb0 = sb ? (b0 >> sb) | (b1 << (64 - sb)) : b0;
             h = ca0 != cb0 || (a1 >> sb) != (b1 >> sb);

@@ -114,7 +114,12 @@ int compute_geo(int64_t n, int d, float eps, Geo &g) {
     g.key_bits = g.bits_o + (d - od) * g.bits_w;
     if (g.key_bits > 128) return EGG_SYNC_E_KEY;
     g.sub_bits = g.key_bits + d <= 128 ? d : 0;
-    g.sort_bits = g.key_bits + g.sub_bits;
+    {
+        // position hash: up to 16 bits, without growing the key past one word if it fits one
+        const int used = g.key_bits + g.sub_bits;
+        g.hash_bits = std::min(16, (used <= 64 ? 64 : 128) - used);
+    }
+    g.sort_bits = g.key_bits + g.sub_bits + g.hash_bits;
     g.kw = g.sort_bits > 64 ? 2 : 1;
     return EGG_SYNC_OK;
 }

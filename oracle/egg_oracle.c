@@ -114,11 +114,18 @@ static int bitlen64(int64_t v) {
     return b < 1 ? 1 : b;
 }
 
-/* Sub-cell ordering is on when the GPU's sort key (cell fields + d split
- * bits) fits 128 bits. */
-static int sub_cells_on(const eo_geom *g) {
-    int key_bits = bitlen64(g->O - 1) + (g->d - g->od) * bitlen64((int64_t)g->w - 1);
-    return key_bits + g->d <= 128;
+/* The GPU's sort key: cell fields, then d split bits if they fit 128 bits,
+ * then up to 16 bits of position hash (not growing a one-word key past 64). */
+static int key_bits_of(const eo_geom *g) {
+    return bitlen64(g->O - 1) + (g->d - g->od) * bitlen64((int64_t)g->w - 1);
+}
+
+static int sub_cells_on(const eo_geom *g) { return key_bits_of(g) + g->d <= 128; }
+
+static int hash_bits_of(const eo_geom *g) {
+    int used = key_bits_of(g) + (sub_cells_on(g) ? g->d : 0);
+    int room = (used <= 64 ? 64 : 128) - used;
+    return room < 16 ? room : 16;
 }
 
 static void geo_f32_from(const eo_geom *g, geo_t_f32 *G) {
@@ -127,6 +134,7 @@ static void geo_f32_from(const eo_geom *g, geo_t_f32 *G) {
     G->eps = g->eps; G->cs = g->cs; G->eps_extra = g->eps_extra;
     G->thr_eps = g->thr_eps; G->thr_half = g->thr_half; G->thr_extra = g->thr_extra;
     G->sub = sub_cells_on(g);
+    G->hash_bits = hash_bits_of(g);
 }
 
 /* ---------------------------------------------------------------- FP64 -- */

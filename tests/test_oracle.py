@@ -103,6 +103,7 @@ def test_grid_invariants(O, d, n, eps, seed):
     X = rng.random((n, d)).astype(f32)
     X[:5] = 1.0  # exercise the w -> w-1 clamp
     X[5:10] = 0.0
+    X[10:20:2] = X[11]  # repeated positions, interleaved with others
     order_in = rng.permutation(n).astype(np.int32)
     G = O.build_grid(X, eps, order_in)
     g = G["geom"]
@@ -118,20 +119,31 @@ def test_grid_invariants(O, d, n, eps, seed):
     keys = [tuple([o] + list(r[g.od:])) for o, r in zip(outer, cg)]
     assert keys == sorted(keys) and len(set(keys)) == len(keys)
     # inside a cell, points are ordered by sub-cell (split bits at the cell
-    # centre, computed in FP32, dimension 0 first), and stably within one
+    # centre, computed in FP32, dimension 0 first), then by the low 16 bits of
+    # a hash of the position's bit patterns, and stably within equal keys;
+    # bitwise-equal positions therefore sit next to each other
     mid = ((coords.astype(f32) + f32(0.5)) * f32(g.cs)).astype(f32)
     sub = (X >= mid).astype(np.int64)
     sub_key = np.zeros(n, np.int64)
     for l in range(d):
         sub_key = sub_key * 2 + sub[:, l]
+    h = np.full(n, 0x9E3779B9, np.uint64)
+    bits = X.view(np.uint32).astype(np.uint64)
+    for l in range(d):
+        h ^= bits[:, l]
+        h = (h * 0x85EBCA6B) & 0xFFFFFFFF
+        h ^= h >> 13
+    h = (h * 0xC2B2AE35) & 0xFFFFFFFF
+    h ^= h >> 16
+    full_key = sub_key * 65536 + (h & 0xFFFF).astype(np.int64)
     rank = np.empty(n, np.int64)
     rank[order_in] = np.arange(n)
     for c in range(G["C"]):
         pts = order[cs_[c]:cs_[c + 1]]
-        sk = sub_key[pts]
-        assert np.all(np.diff(sk) >= 0)
-        for v in np.unique(sk):
-            assert np.all(np.diff(rank[pts[sk == v]]) > 0)
+        fk = full_key[pts]
+        assert np.all(np.diff(fk) >= 0)
+        for v in np.unique(fk):
+            assert np.all(np.diff(rank[pts[fk == v]]) > 0)
     for o in np.unique(outer):
         cells = np.nonzero(outer == o)[0]
         assert G["ob"][o] == cells[0] and G["oe"][o] == cells[-1] + 1
