@@ -284,7 +284,7 @@ __device__ __forceinline__ int sub_class(const int *__restrict__ sub_g, int n, i
 // LDS in chunks of 64 runs -- each lane loads one run and computes sin v and
 // cos v - 1 of v = x_q - a0 (a0 = the tile's anchor, near the tile's cells) --
 // and every lane reads each run as a broadcast and, for runs within eps,
-// accumulates sin v and cos v - 1 (once per point of the run).  At the end of
+// accumulates sin v and cos v - 1 times the run's length.  At the end of
 // the tile, sum_q sin(x_q - x) = cos u * sum sin v - sin u * (m + sum (cos v - 1)),
 // u = x - a0, m = number of such points.  All angles stay within a few cell
 // sides of the anchor, and both sums grow like the sum of sin(x_q - x) does,
@@ -352,8 +352,8 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         // Streams runs through LDS, 64 per chunk: runs [r0, r1) if smask is
         // 0, else the runs of the sub-cells s0 + k of the bits k of smask, in
         // order.  Lanes with `acc` test each run against eps and accumulate
-        // it (counted with its length; a run's terms are added once per
-        // point, in order, as a pointwise loop would).
+        // it, weighted by its length (one rounding per run instead of one per
+        // point).
         auto scan_runs = [&](int r0, int r1, int s0, unsigned smask,
                              bool acc) __attribute__((always_inline)) {
             const bool any_acc = __any(acc);
@@ -452,13 +452,13 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                         }
                         if (sq <= thr) {
                             const int lk = len[k];
+                            const float w = (float)lk;  // exact: lk < 2^24
 #pragma unroll
                             for (int l = 0; l < D; l++) {
-                                const float a = qq.sv[l], b = qq.cv[l];
-                                for (int e = 0; e < lk; e++) {
-                                    ssv[l] += a;
-                                    scm[l] += b;
-                                }
+                                const float a = w * qq.sv[l];
+                                const float b = w * qq.cv[l];
+                                ssv[l] += a;
+                                scm[l] += b;
                             }
                             m_part += lk;
                         }
