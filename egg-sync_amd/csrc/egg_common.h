@@ -29,9 +29,6 @@ constexpr int kSumChunk = 2048;   // max points per summary work item
 constexpr int kTile = 64;         // max runs per update tile (one wavefront)
 constexpr int kDefaultGroup = 1;  // default cells per tile group along the last coordinate
 constexpr int kAnchorBlock = 8;   // group widths are powers of two dividing this
-constexpr int kHeavyRuns = 16384; // update tiles streaming more runs from one cell go cooperative
-constexpr int kCoopWaves = 8;     // wavefronts per cooperative update tile
-constexpr int kCoopBlocks = 512;  // workgroups of the cooperative update pass
 
 // Grid geometry and thresholds, computed on the host (EGG_SynC.cu:1249-1260).
 struct Geo {
@@ -107,8 +104,6 @@ struct DevState {
     int runs;          // runs of equal positions in the sorted order
     int G;             // cell groups of the update tiles
     int S;             // non-empty sub-cells
-    int heavy_count;   // update tiles deferred to the cooperative pass
-    int heavy_next;    // work-queue head of that pass
     int WS;            // sub-cell summary work items
     double rc;         // order parameter of the last r_c evaluation
 };

@@ -58,7 +58,6 @@ struct GridBufs {
     int *grp_cell;       // [n+1] first cell of each group, grp_cell[G] = C
     int *ntile, *tile_off;// [n]  tiles per group, exclusive scan
     float *rres;         // [d][n] new position of each run
-    int *heavy;          // [n]   update tiles deferred to the cooperative pass
     int *rdbg;           // [4n]  N / half / border / synced of each run (diagnostics)
     float *Ts, *Tc;      // [d][n]
     float *partial;      // [n + n/kSumChunk + 1][4*d]

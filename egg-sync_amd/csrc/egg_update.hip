@@ -291,17 +291,8 @@ __device__ __forceinline__ int sub_class(const int *__restrict__ sub_g, int n, i
 // so the result keeps the accuracy of a direct sum of sinf(x_q - x).  Border
 // lanes count their eps/2-neighbours in other full cells from the same
 // chunks (EGG_SynC.cu:666-681).
-//
-// Two phases.  COOP = false: one wavefront per tile from the work queue; a
-// tile that would stream more than kHeavyRuns runs from one cell is put on
-// the heavy list instead and left unfinished.  COOP = true: one workgroup of
-// kCoopWaves wavefronts per heavy tile; all walk the candidate cells, chunk k
-// of the tile's streams is processed by wavefront k mod kCoopWaves, the cell
-// and sub-cell totals by wavefront 0, and the partial sums are combined in
-// wavefront order -- so a few targets next to a dense, not yet collapsed
-// cluster no longer hold up the whole pass.
-template <int D, bool COOP>
-__global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_tile(
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_update_tile(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
     const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ run_start,
@@ -310,18 +301,10 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
     const int *__restrict__ cell_sub, const int *__restrict__ sub_start,
     const int *__restrict__ sub_g, const int *__restrict__ sub_run,
     const float *__restrict__ TsS, const float *__restrict__ TcS,
-    float *__restrict__ rres, int *__restrict__ rdbg, int *__restrict__ heavy, Geo g,
-    DevState *st) {
+    float *__restrict__ rres, int *__restrict__ rdbg, Geo g, DevState *st) {
     if (st->done) return;
-    if (COOP && st->heavy_count == 0) return;
-    constexpr int NW = COOP ? kCoopWaves : kBlock / kWave;
-    __shared__ SrcPt<D> s_src[NW][kWave];
-    __shared__ int s_len[NW][kWave];
-    __shared__ float s_acc[COOP ? kCoopWaves : 1][3 * D][kWave];  // sum, ssv, scm
-    __shared__ int s_cnt[COOP ? kCoopWaves : 1][3][kWave];        // N, half, m_part
-    __shared__ int s_item;
-    const int wv = threadIdx.x / kWave;
-    const int part = COOP ? wv : 0, nparts = COOP ? kCoopWaves : 1;
+    __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
+    __shared__ int s_len[kBlock / kWave][kWave];
     const int n = g.n, R = g.R, od = g.od;
     const int G = st->G, T = st->T;
     const float thr = g.thr_eps, thr_half = g.thr_half;
@@ -336,18 +319,8 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
     const int *cg_od = cell_g + (long long)od * n;
     bool unsynced = false;
     for (;;) {
-        int t;
-        if (COOP) {
-            if (threadIdx.x == 0) s_item = atomicAdd(&st->heavy_next, 1);
-            __syncthreads();
-            const int item = s_item;
-            __syncthreads();
-            if (item >= st->heavy_count) break;
-            t = heavy[item];
-        } else {
-            t = next_tile(&st->tile_next);
-            if (t >= T) break;
-        }
+        const int t = next_tile(&st->tile_next);
+        if (t >= T) break;
         const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
         const int c0 = tl.gc0, cl = tl.cl, r = tl.r, i = tl.i;
         const bool active = tl.active;
@@ -376,8 +349,6 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
         }
         const int g0od = select_dim<D>(g0, od);
         int N = 0, half = 0, m_part = 0;
-        bool too_heavy = false;  // phase 1: a stream above kHeavyRuns was found
-        int chunk_ctr = 0;       // chunks of this tile's streams, in order
         // Streams runs through LDS, 64 per chunk: runs [r0, r1) if smask is
         // 0, else the runs of the sub-cells s0 + k of the bits k of smask, in
         // order.  Lanes with `acc` test each run against eps and accumulate
@@ -394,12 +365,7 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
                     total += sub_run[s0 + k + 1] - sub_run[s0 + k];
                 }
             }
-            if (!COOP && total > kHeavyRuns) {
-                too_heavy = true;
-                return;
-            }
             for (int base = 0; base < total; base += kWave) {
-                if (COOP && (chunk_ctr++) % nparts != part) continue;
                 const int q = base + lane;
                 int rr = -1;
                 if (!smask) {
@@ -528,9 +494,8 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
             }
         };
         walk_columns<D>(g0, R, g, ob, oe, [&](int cb, const int ce) {
-            if (too_heavy) return;
             if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
-            for (int c = cb; c < ce && !too_heavy; c++) {
+            for (int c = cb; c < ce; c++) {
                 int gc[D];
                 bool stop;
                 const bool in =
@@ -540,8 +505,8 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
                 float sm, lg;
                 box_dist<D>(gc, x, g.cs, sm, lg);
                 const bool full = active && lg <= thr;
-                const bool lpart = active && !full && sm <= thr;
-                if (full && part == 0) {
+                const bool part = active && !full && sm <= thr;
+                if (full) {
 #pragma unroll
                     for (int l = 0; l < D; l++) {
                         const float a = Ts[(long long)l * n + c] * cx[l];
@@ -553,8 +518,8 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
                     N += cnt;
                     if (border && c == cl) half += cnt;
                 }
-                const bool need_half = full && border && c != cl && part == 0;
-                const bool any_part = __any(lpart), any_half = __any(need_half);
+                const bool need_half = full && border && c != cl;
+                const bool any_part = __any(part), any_half = __any(need_half);
                 if (!any_part && !any_half) continue;
                 if (any_half) half_scan(cell_run[c], cell_run[c + 1], need_half);
                 if (!any_part) continue;
@@ -568,7 +533,7 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
                     // a partial cell; other sub-cells wholly within eps add
                     // their totals
                     unsigned imask = 0, smask = 0;
-                    if (lpart) {
+                    if (part) {
                         for (int k = 0; k < ns; k++) {
                             const int cls = sub_class<D>(sub_g, n, s0 + k, x, g, thr_in, thr_out);
                             if (cls == 1) imask |= 1u << k;
@@ -580,7 +545,7 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
                     for (unsigned mm = wave_or(tmask); mm; mm &= mm - 1) {
                         const int k = __builtin_ctz(mm);
                         const int sc = s0 + k;
-                        if (((tmask >> k) & 1u) && part == 0) {
+                        if ((tmask >> k) & 1u) {
 #pragma unroll
                             for (int l = 0; l < D; l++) {
                                 const float a = TsS[(long long)l * n + sc] * cx[l];
@@ -593,37 +558,9 @@ __global__ __launch_bounds__(COOP ? kCoopWaves * kWave : kBlock) void k_update_t
                     }
                     if (!umask) continue;
                 }
-                scan_runs(cell_run[c], cell_run[c + 1], s0, umask, lpart);
+                scan_runs(cell_run[c], cell_run[c + 1], s0, umask, part);
             }
         });
-        if (!COOP && too_heavy) {
-            if (lane == 0) heavy[atomicAdd(&st->heavy_count, 1)] = t;
-            continue;
-        }
-        if (COOP) {
-#pragma unroll
-            for (int l = 0; l < D; l++) {
-                s_acc[wv][l][lane] = sum[l];
-                s_acc[wv][D + l][lane] = ssv[l];
-                s_acc[wv][2 * D + l][lane] = scm[l];
-            }
-            s_cnt[wv][0][lane] = N;
-            s_cnt[wv][1][lane] = half;
-            s_cnt[wv][2][lane] = m_part;
-            __syncthreads();
-            if (wv > 0) continue;
-            for (int w = 1; w < kCoopWaves; w++) {
-#pragma unroll
-                for (int l = 0; l < D; l++) {
-                    sum[l] += s_acc[w][l][lane];
-                    ssv[l] += s_acc[w][D + l][lane];
-                    scm[l] += s_acc[w][2 * D + l][lane];
-                }
-                N += s_cnt[w][0][lane];
-                half += s_cnt[w][1][lane];
-                m_part += s_cnt[w][2][lane];
-            }
-        }
         N += m_part;
         float xo[D];
 #pragma unroll
@@ -1019,8 +956,6 @@ __global__ void k_set_flag(DevState *st) {
     st->pair_count = 0;
     st->tile_next = 0;
     st->xtile_next = 0;
-    st->heavy_count = 0;
-    st->heavy_next = 0;
 }
 
 // End of an iteration (EGG_SynC.cu:1322,1425 / SynC.cpp:277).
@@ -1067,14 +1002,11 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
     int *rdbg = (d.N || d.half || d.border || d.synced) ? b.rdbg : nullptr;
 #define UPD_CASE(DD)                                                                          \
     case DD:                                                                                  \
-        k_update_tile<DD, false><<<qb, qt, 0, s>>>(                                         \
-            b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.ob, b.oe, b.run_start, b.cell_run,      \
-            b.grp_cell, b.point_cell, b.tile_off, b.cell_sub, b.sub_start, b.sub_g, b.sub_run,  \
-            b.TsS, b.TcS, b.rres, rdbg, b.heavy, g, st);                                        \
-        k_update_tile<DD, true><<<kCoopBlocks, kCoopWaves * kWave, 0, s>>>(                  \
-            b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.ob, b.oe, b.run_start, b.cell_run,      \
-            b.grp_cell, b.point_cell, b.tile_off, b.cell_sub, b.sub_start, b.sub_g, b.sub_run,  \
-            b.TsS, b.TcS, b.rres, rdbg, b.heavy, g, st);                                        \
+        k_update_tile<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc,     \
+                                            b.ob, b.oe, b.run_start, b.cell_run,              \
+                                            b.grp_cell, b.point_cell, b.tile_off,             \
+                                            b.cell_sub, b.sub_start, b.sub_g, b.sub_run,      \
+                                            b.TsS, b.TcS, b.rres, rdbg, g, st);               \
         k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.head, b.hpos, b.rres, rdbg,     \
                                                             b.perm_s, b.X, b.perm, g.n, st,   \
                                                             d);                               \
