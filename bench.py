@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # the counter reads for a 256 MiB copy (2x and 4x the bytes moved); see
 # profiles/k_update_tile_config1_n1M_counters.md.  Measured for this workload
 # only, so reported only when n is 1,000,000.
-UPDATE_TRAFFIC_BYTES_N1M = 53.5e6
+UPDATE_TRAFFIC_BYTES_N1M = 47.8e6
 
 
 def cpu_baseline(X, eps, target_seconds=12.0):
