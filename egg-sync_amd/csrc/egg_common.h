@@ -29,6 +29,7 @@ constexpr int kSumChunk = 2048;   // max points per summary work item
 constexpr int kTile = 64;         // max runs per update tile (one wavefront)
 constexpr int kDefaultGroup = 1;  // default cells per tile group along the last coordinate
 constexpr int kAnchorBlock = 8;   // group widths are powers of two dividing this
+constexpr int kSparseRuns = 8;    // cell groups with fewer runs are updated one lane per run
 
 // Grid geometry and thresholds, computed on the host (EGG_SynC.cu:1249-1260).
 struct Geo {
@@ -104,6 +105,7 @@ struct DevState {
     int runs;          // runs of equal positions in the sorted order
     int G;             // cell groups of the update tiles
     int S;             // non-empty sub-cells
+    int sparse_count;  // runs of sparse cell groups (k_update_sparse)
     int WS;            // sub-cell summary work items
     double rc;         // order parameter of the last r_c evaluation
 };

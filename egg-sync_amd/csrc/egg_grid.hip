@@ -512,6 +512,36 @@ __global__ void k_store_total(const int *__restrict__ off, const int *__restrict
     *out = m > 0 ? off[m - 1] + cnt[m - 1] : 0;
 }
 
+// Runs of the sparse groups (fewer than kSparseRuns runs), listed for
+// k_update_sparse: per group the count (0 for dense ones), then the list.
+__global__ __launch_bounds__(kBlock) void k_sparse_count(const int *__restrict__ cell_run,
+                                                         const int *__restrict__ grp_cell,
+                                                         const DevState *st,
+                                                         int *__restrict__ cnt,
+                                                         DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int G = st->G;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
+        const int runs = cell_run[grp_cell[k + 1]] - cell_run[grp_cell[k]];
+        cnt[k] = runs < kSparseRuns ? runs : 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sparse_list(const int *__restrict__ cell_run,
+                                                        const int *__restrict__ grp_cell,
+                                                        const int *__restrict__ off,
+                                                        const DevState *st,
+                                                        int *__restrict__ list,
+                                                        DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int G = st->G;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
+        const int r0 = cell_run[grp_cell[k]], runs = cell_run[grp_cell[k + 1]] - r0;
+        if (runs < kSparseRuns)
+            for (int j = 0; j < runs; j++) list[off[k] + j] = r0 + j;
+    }
+}
+
 // First run of every sub-cell (a sub-cell starts where the position changes,
 // so its first sorted position heads a run; hpos holds the run-head scan).
 __global__ __launch_bounds__(kBlock) void k_sub_runs(const int *__restrict__ sub_start,
@@ -575,6 +605,12 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     k_tile_count<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.ntile, st_loop);
     scan_exclusive(b.ntile, b.tile_off, n, &st->G, b.scan_tmp, nullptr, st_loop, s);
     k_store_total<<<1, 1, 0, s>>>(b.tile_off, b.ntile, &st->G, &st->T, st_loop);
+    // the runs of sparse groups, for the one-lane-per-run update
+    k_sparse_count<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.sparse_runs, st_loop);
+    scan_exclusive(b.sparse_runs, b.sparse_off, n, &st->G, b.scan_tmp, &st->sparse_count,
+                   st_loop, s);
+    k_sparse_list<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, b.sparse_off, st,
+                                        b.sparse_runs, st_loop);
     if (g.sub_bits) {
         // sub-cell runs and totals (anchors on the half-side grid)
         k_sub_runs<<<gb, kBlock, 0, s>>>(b.sub_start, b.hpos, st, b.sub_run, st_loop);

@@ -58,6 +58,8 @@ struct GridBufs {
     int *grp_cell;       // [n+1] first cell of each group, grp_cell[G] = C
     int *ntile, *tile_off;// [n]  tiles per group, exclusive scan
     float *rres;         // [d][n] new position of each run
+    int *sparse_runs;    // [n]   runs of sparse cell groups
+    int *sparse_off;     // [n]   their offsets per group (exclusive scan)
     int *rdbg;           // [4n]  N / half / border / synced of each run (diagnostics)
     float *Ts, *Tc;      // [d][n]
     float *partial;      // [n + n/kSumChunk + 1][4*d]

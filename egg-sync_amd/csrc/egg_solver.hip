@@ -193,6 +193,8 @@ Layout carve(void *base, const Geo &g) {
     b.ntile = c.take<int>(n);
     b.tile_off = c.take<int>(n);
     b.rres = c.take<float>(n * d);
+    b.sparse_runs = c.take<int>(n);
+    b.sparse_off = c.take<int>(n);
     b.rdbg = c.take<int>(4 * n);
     b.Ts = c.take<float>(n * d);
     b.Tc = c.take<float>(n * d);

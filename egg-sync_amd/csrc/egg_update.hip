@@ -322,6 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         const int t = next_tile(&st->tile_next);
         if (t >= T) break;
         const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
+        if (cell_run[tl.gc1] - cell_run[tl.gc0] < kSparseRuns) continue;  // k_update_sparse
         const int c0 = tl.gc0, cl = tl.cl, r = tl.r, i = tl.i;
         const bool active = tl.active;
         const int own = cell_start[cl + 1] - cell_start[cl];
@@ -586,6 +587,119 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         }
     }
     if (__any(unsynced) && lane == 0) atomicAnd(&st->flag, 0);
+}
+
+// The update for the runs of sparse cell groups (fewer than kSparseRuns runs;
+// the tile kernel skips those groups): one lane per run, each walking its own
+// candidate cells (the Odo order), full cells through their totals, partial
+// cells run by run with sinf(x_q - x) weighted by the run's length, as
+// EGG_SynC.cu:640-703 writes it.  Nearly empty cells -- all cells of sparse
+// high-dimensional data, converged clusters -- would leave a tile's
+// wavefront with a lane or two; here every lane has a point.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_update_sparse(
+    const float *__restrict__ Xs, const int *__restrict__ cell_start,
+    const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
+    const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ run_start,
+    const int *__restrict__ cell_run, const int *__restrict__ point_cell,
+    const int *__restrict__ sparse_runs, float *__restrict__ rres, int *__restrict__ rdbg,
+    Geo g, DevState *st) {
+    if (st->done) return;
+    const int n = g.n, od = g.od, R = g.R, count = st->sparse_count;
+    const float thr = g.thr_eps, thr_half = g.thr_half;
+    const int *cg_od = cell_g + (long long)od * n;
+    bool unsynced = false;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+        const int r = sparse_runs[k];
+        const int i = run_start[r];
+        const int c0 = point_cell[i];
+        const int own = cell_start[c0 + 1] - cell_start[c0];
+        float x[D], sx[D], cx[D], sum[D], sml[D];
+        int g0[D];
+        bool border = false;
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            x[l] = Xs[(long long)l * n + i];
+            g0[l] = cell_g[(long long)l * n + c0];
+            sx[l] = sinf(x[l]);
+            cx[l] = cosf(x[l]);
+            sum[l] = 0.f;
+            sml[l] = 0.f;
+            border |= on_border(x[l], g.cs);
+        }
+        const int g0od = select_dim<D>(g0, od);
+        int N = 0, half = 0;
+        Odo<D> odo;
+        odo.init(od, R);
+        while (!odo.finished) {
+            const int o = odo.outer(g0, g);
+            if (o >= 0) {
+                int cb = ob[o];
+                const int ce = oe[o];
+                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
+                for (int c = cb; c < ce; c++) {
+                    int gc[D];
+                    bool stop;
+                    const bool in = cell_in_window<D>(cell_g, n, c, g0, od, R + 1, g0od, gc, &stop);
+                    if (stop) break;
+                    if (!in) continue;
+                    float sm, lg;
+                    box_dist<D>(gc, x, g.cs, sm, lg);
+                    if (lg <= thr) {
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            const float a = Ts[(long long)l * n + c] * cx[l];
+                            const float b = Tc[(long long)l * n + c] * sx[l];
+                            const float tt = a - b;
+                            sum[l] += tt;
+                        }
+                        const int cnt = cell_start[c + 1] - cell_start[c];
+                        N += cnt;
+                        if (border && c == c0) half += cnt;
+                        if (border && c != c0) {
+                            for (int q = cell_run[c]; q < cell_run[c + 1]; q++) {
+                                const int j = run_start[q];
+                                float xq[D];
+                                if (dist2_to<D>(x, Xs, n, j, xq) <= thr_half)
+                                    half += run_start[q + 1] - j;
+                            }
+                        }
+                    } else if (sm <= thr) {
+                        for (int q = cell_run[c]; q < cell_run[c + 1]; q++) {
+                            const int j = run_start[q];
+                            float xq[D];
+                            if (dist2_to<D>(x, Xs, n, j, xq) <= thr) {
+                                const int lq = run_start[q + 1] - j;
+                                const float w = (float)lq;  // exact: lq < 2^24
+#pragma unroll
+                                for (int l = 0; l < D; l++) sml[l] += w * sinf(xq[l] - x[l]);
+                                N += lq;
+                            }
+                        }
+                    }
+                }
+            }
+            odo.next(od, R);
+        }
+        float xo[D];
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            float tt = sum[l] + sml[l];
+            tt = tt / (float)N;
+            xo[l] = x[l] + tt;
+        }
+        const bool synced = !((!border && N != own) || (border && N != half));
+#pragma unroll
+        for (int l = 0; l < D; l++) rres[(long long)l * n + r] = xo[l];
+        if (!synced) unsynced = true;
+        if (rdbg) {
+            rdbg[4 * (long long)r + 0] = N;
+            rdbg[4 * (long long)r + 1] = half;
+            rdbg[4 * (long long)r + 2] = border ? 1 : 0;
+            rdbg[4 * (long long)r + 3] = synced ? 1 : 0;
+        }
+    }
+    if (__any(unsynced) && __lane_id() == 0) atomicAnd(&st->flag, 0);
 }
 
 // Every point takes its run's result and carries its id over to the next
@@ -1007,6 +1121,9 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
                                             b.grp_cell, b.point_cell, b.tile_off,             \
                                             b.cell_sub, b.sub_start, b.sub_g, b.sub_run,      \
                                             b.TsS, b.TcS, b.rres, rdbg, g, st);               \
+        k_update_sparse<DD><<<grid_for(g.n), kBlock, 0, s>>>(                                \
+            b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.ob, b.oe, b.run_start, b.cell_run,      \
+            b.point_cell, b.sparse_runs, b.rres, rdbg, g, st);                                 \
         k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.head, b.hpos, b.rres, rdbg,     \
                                                             b.perm_s, b.X, b.perm, g.n, st,   \
                                                             d);                               \
