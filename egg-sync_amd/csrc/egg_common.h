@@ -29,7 +29,15 @@ constexpr int kSumChunk = 2048;   // max points per summary work item
 constexpr int kTile = 64;         // max runs per update tile (one wavefront)
 constexpr int kDefaultGroup = 1;  // default cells per tile group along the last coordinate
 constexpr int kAnchorBlock = 8;   // group widths are powers of two dividing this
-constexpr int kSparseRuns = 8;    // cell groups with fewer runs are updated one lane per run
+constexpr int kSparseRuns = 8;    // cell groups with fewer runs are updated one lane per run,
+constexpr int kSparseFactor = 4;  // in iterations with fewer than kSparseFactor runs per cell
+
+// Whether the cell groups with `runs` runs go to the one-lane-per-run update
+// in this iteration: only when the grid is sparse overall, where streaming a
+// neighbour cell per lane is cheap.
+__host__ __device__ __forceinline__ bool sparse_group(int runs, int all_runs, int cells) {
+    return runs < kSparseRuns && (long long)all_runs < (long long)kSparseFactor * cells;
+}
 
 // Grid geometry and thresholds, computed on the host (EGG_SynC.cu:1249-1260).
 struct Geo {

@@ -512,8 +512,8 @@ __global__ void k_store_total(const int *__restrict__ off, const int *__restrict
     *out = m > 0 ? off[m - 1] + cnt[m - 1] : 0;
 }
 
-// Runs of the sparse groups (fewer than kSparseRuns runs), listed for
-// k_update_sparse: per group the count (0 for dense ones), then the list.
+// Runs of the sparse groups (sparse_group), listed for k_update_sparse: per
+// group the count (0 for the others), then the list.
 __global__ __launch_bounds__(kBlock) void k_sparse_count(const int *__restrict__ cell_run,
                                                          const int *__restrict__ grp_cell,
                                                          const DevState *st,
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_count(const int *__restrict__
     const int G = st->G;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
         const int runs = cell_run[grp_cell[k + 1]] - cell_run[grp_cell[k]];
-        cnt[k] = runs < kSparseRuns ? runs : 0;
+        cnt[k] = sparse_group(runs, st->runs, st->C) ? runs : 0;
     }
 }
 
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_list(const int *__restrict__ 
     const int G = st->G;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
         const int r0 = cell_run[grp_cell[k]], runs = cell_run[grp_cell[k + 1]] - r0;
-        if (runs < kSparseRuns)
+        if (sparse_group(runs, st->runs, st->C))
             for (int j = 0; j < runs; j++) list[off[k] + j] = r0 + j;
     }
 }

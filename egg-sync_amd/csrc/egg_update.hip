@@ -322,7 +322,8 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         const int t = next_tile(&st->tile_next);
         if (t >= T) break;
         const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
-        if (cell_run[tl.gc1] - cell_run[tl.gc0] < kSparseRuns) continue;  // k_update_sparse
+        if (sparse_group(cell_run[tl.gc1] - cell_run[tl.gc0], st->runs, st->C))
+            continue;  // k_update_sparse
         const int c0 = tl.gc0, cl = tl.cl, r = tl.r, i = tl.i;
         const bool active = tl.active;
         const int own = cell_start[cl + 1] - cell_start[cl];
@@ -589,8 +590,8 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     if (__any(unsynced) && lane == 0) atomicAnd(&st->flag, 0);
 }
 
-// The update for the runs of sparse cell groups (fewer than kSparseRuns runs;
-// the tile kernel skips those groups): one lane per run, each walking its own
+// The update for the runs of sparse cell groups (sparse_group; the tile
+// kernel skips those groups): one lane per run, each walking its own
 // candidate cells (the Odo order), full cells through their totals, partial
 // cells run by run with sinf(x_q - x) weighted by the run's length, as
 // EGG_SynC.cu:640-703 writes it.  Nearly empty cells -- all cells of sparse
