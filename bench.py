@@ -11,13 +11,14 @@ the timed steps.  The CPU baseline is the reference's own CPU SynC
 (src/algorithms/SynC.cpp, compiled from source into oracle/_ref) running one
 OpenMP pass over a bounded sample of target points of the same data.
 
-The line also carries a "forms" object: the learned-jump form on n=4M,
-d=3, eps=0.01 (trained weights from egg_sync/weights, plain solve of the same
-data timed beside it, labels compared) and the batched form (64 datasets x 4
-eps values = 256 solves in one call).
+The line also carries a "forms" object: the plain solve of the n=250k, d=8,
+eps=0.1 workload; the learned-jump form on n=4M, d=3, eps=0.01 (trained
+weights from egg_sync/weights, plain solve of the same data timed beside it,
+labels compared); the batched form (64 datasets x 4 eps values = 256 solves
+in one call).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
-                    [--forms plain,learned,batched]
+                    [--forms plain,plain_d8,learned,batched]
 """
 import argparse
 import json
@@ -116,6 +117,24 @@ def learned_form(n=4_000_000):
             "result_used": "learned" if same else "plain (fallback)"}
 
 
+def plain_d8_form():
+    """The plain solve on the n=250,000, d=8, eps=0.1 workload (sparse 8-d grid)."""
+    import torch
+    from egg_sync import data, solver
+    X, eps = data.config2()
+    D = torch.from_numpy(X).cuda()
+    solver.solve(D, eps, return_positions=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = solver.solve(D, eps, return_positions=False)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    it = r["stats"]["iterations"]
+    return {"workload": f"n={len(X)}, d=8, eps={eps}", "time_to_clustering_s": t,
+            "iterations": it, "point_updates_per_s": len(X) * it / t,
+            "clusters": r["stats"]["n_clusters"]}
+
+
 def batched_form(count=64):
     """256 solves (64 datasets x 4 eps) in one batched call vs one call each."""
     import torch
@@ -175,7 +194,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--iters-per-sync", type=int, default=8)
-    ap.add_argument("--forms", default="plain,learned,batched",
+    ap.add_argument("--forms", default="plain,plain_d8,learned,batched",
                     help="comma list; plain is always measured")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known traffic "
@@ -268,7 +287,8 @@ def main():
         line["roofline"]["copy_gbs_measured"] = None
         print(f"copy bandwidth: {e}", file=sys.stderr)
     forms = {}
-    for name, fn in (("learned", learned_form), ("batched", batched_form)):
+    for name, fn in (("plain_d8", plain_d8_form), ("learned", learned_form),
+                     ("batched", batched_form)):
         if name in args.forms.split(","):
             try:
                 forms[name] = fn()
