@@ -100,16 +100,19 @@ def learned_form(n=4_000_000):
     print("learned form: warm-up", file=sys.stderr, flush=True)
     X, eps = data.config4(n=n)
     D = torch.from_numpy(X).cuda()
-    learned.solve_learned(D, eps, net)          # warm-up: workspace, code objects
+    learned.solve_learned(D, eps, net, t0=1, alpha=0.5, clip=0.5)  # warm-up
     solver.solve(D, eps, return_positions=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     p = solver.solve(D, eps, return_positions=False)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    r = learned.solve_learned(D, eps, net)
+    r = learned.solve_learned(D, eps, net, t0=1, alpha=0.5, clip=0.5)
     same = bool(same_partition(r["labels"], p["labels"]))
-    return {"workload": f"n={n}, d=3, eps={eps}", "plain_s": t_plain,
+    from egg_sync.metrics import disagreement
+    return {"workload": f"n={n}, d=3, eps={eps}", "jump": "t0=1, alpha=0.5, clip=0.5",
+            "disagreement_with_plain": disagreement(r["labels"], p["labels"]),
+            "plain_s": t_plain,
             "plain_iterations": p["stats"]["iterations"],
             "plain_point_updates_per_s": n * p["stats"]["iterations"] / t_plain,
             "learned_s": r["seconds"], "learned_passes": r["iterations"],

@@ -5,6 +5,10 @@ with the CPU oracle without re-running it:
                       labels (int32, point-id order), iteration count, a seeded
                       sample of 20,000 final positions, column sums of all final
                       positions.
+  c3_tail_oracle.npz  the slowest instance of the batched workload (dataset 27 of the
+                      64-instance generator, n=165,583, d=2, eps=0.1; ~11,600
+                      iterations, ~3 min): labels, iteration count, sampled final
+                      positions.
   c2_full_oracle.npz  config 2 (n=250,000, d=8, eps=0.1; ~3 min): the same from the
                       FP32 oracle, plus the FP64 oracle's iteration count, labels
                       and sampled final positions, and the order parameter r_c of
@@ -67,8 +71,20 @@ def c2():
     print("c2 clusters", int(r["labels"].max()) + 1, "rc64", rc64, flush=True)
 
 
+def c3_tail():
+    X = data.config3_instances(seed=3, count=64)[27]
+    r = O.egg_solve(X, 0.1)
+    rng = np.random.default_rng(0)
+    idx = np.sort(rng.choice(len(X), 5000, replace=False)).astype(np.int32)
+    np.savez_compressed(os.path.join(HERE, "c3_tail_oracle.npz"), labels=r["labels"],
+                        iterations=np.int32(r["iterations"]), sample_idx=idx,
+                        sample_X=r["X"][idx])
+    print("c3 tail iterations", r["iterations"], "clusters", int(r["labels"].max()) + 1,
+          flush=True)
+
+
 if __name__ == "__main__":
     O.build()
-    todo = sys.argv[1:] or ["c1", "c2"]
+    todo = sys.argv[1:] or ["c1", "c2", "c3_tail"]
     for name in todo:
-        {"c1": c1, "c2": c2}[name]()
+        {"c1": c1, "c2": c2, "c3_tail": c3_tail}[name]()

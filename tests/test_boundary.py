@@ -163,3 +163,15 @@ int main() {
                                  check=True).stdout.split()
     assert float(ws) <= 1.5 and float(wc) <= 1.5, (ws, wc)
     assert big == "1"
+
+
+def test_partition_metrics():
+    from egg_sync.metrics import disagreement, same_partition
+    a = np.array([0, 0, 1, 1, -1, 2, 2])
+    assert same_partition(a, np.array([3, 3, 0, 0, -1, 9, 9]))
+    assert not same_partition(a, np.array([3, 3, 0, 0, 4, 9, 9]))      # noise differs
+    b = np.array([5, 5, 7, 7, -1, 7, 7])                                # two clusters merged
+    assert not same_partition(a, b)
+    assert disagreement(a, b) == {"noise_differs": 0, "unmatched": 2, "clusters_a": 3,
+                                  "clusters_b": 2}
+    assert disagreement(a, a)["unmatched"] == 0
