@@ -175,3 +175,28 @@ def test_partition_metrics():
     assert disagreement(a, b) == {"noise_differs": 0, "unmatched": 2, "clusters_a": 3,
                                   "clusters_b": 2}
     assert disagreement(a, a)["unmatched"] == 0
+
+
+def test_experiment_data_recipe():
+    from egg_sync import experiments as E
+    X = E.synthetic_gauss(5000, 3, 5, 5.0, seed=1)
+    assert X.shape == (5000, 3) and X.dtype == np.float32
+    assert X.min() == 0.0 and X.max() == 1.0
+    assert np.array_equal(X, E.synthetic_gauss(5000, 3, 5, 5.0, seed=1))
+    assert not np.array_equal(X, E.synthetic_gauss(5000, 3, 5, 5.0, seed=2))
+    assert E.STAGES == ["Allocating", "Build structure", "Update", "Extra check", "Clustering",
+                        "Free Memory"]
+    assert set(E.SWEEPS) == {"n", "d", "std", "cl", "eps"}
+
+
+def test_dataset_loaders_read_the_reference_files():
+    """The loaders read the reference's own data files (python/data.py:46-97)."""
+    from egg_sync import experiments as E
+    data_dir = "/root/reference/data"
+    if not os.path.isdir(data_dir):
+        pytest.skip("reference data files not present")
+    for name, (fname, _) in E.DATASETS.items():
+        if not os.path.exists(os.path.join(data_dir, fname)):
+            continue
+        X = E.load_dataset(name, data_dir)
+        assert X.ndim == 2 and X.shape[0] > 100 and 0.0 <= X.min() and X.max() <= 1.0, name

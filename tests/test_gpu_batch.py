@@ -180,3 +180,15 @@ def test_autotuner_checks_results_and_caches(S, tmp_path, monkeypatch):
     assert autotune.cached(autotune.batch_signature(specs)) == best
     r = S.solve_batch(D, specs, **best)
     assert r["results"][0]["iterations"] > 0
+
+
+def test_experiment_harness_records():
+    from egg_sync import experiments as E
+    recs = E.run("n", [2000, 4000], rounds=2, d=2, eps=0.05)
+    assert len(recs) == 4
+    for r in recs:
+        assert r["running_time"] > 0 and r["iterations"] >= 1 and r["space"] > 0
+        assert list(r["stage_times"]) == E.STAGES
+        assert len(r["itr_times"]) == r["iterations"]
+    recs = E.run("d", [3, 8], rounds=1, n=3000, eps=0.2)
+    assert [r["d"] for r in recs] == [3, 8]
