@@ -5,7 +5,8 @@
 //                             the last grid build (state between iterations)
 //   Xs[d][n]           float  positions sorted by cell (this iteration)
 //   perm[n], perm_s[n] int32  point id at each position
-//   keys[KW][n]        u64    cell key (outer id, g_od, ..., g_{d-1}) bit fields
+//   keys[KW][n]        u64    sort key (outer id, g_od, ..., g_{d-1}, split bits,
+//                             position hash), KW <= kMaxKW words, low word first
 //   cell_start[C+1], cell_g[d][n], cell_outer[n], Ts[d][n], Tc[d][n]
 //   ob[O], oe[O]       int32  cell range of each outer cell
 // All FP arithmetic that decides membership is done exactly as the
@@ -18,7 +19,8 @@
 
 namespace egg {
 
-constexpr int kMaxD = 8;          // dimensions supported by the device kernels
+constexpr int kMaxD = 20;         // dimensions supported by the device kernels
+constexpr int kMaxKW = 4;         // 64-bit words of a sort key (keys up to 256 bits)
 constexpr int kBlock = 256;       // threads per block (4 wavefronts)
 constexpr int kWave = 64;
 constexpr int kScanItems = 8;     // items per thread in scans
@@ -31,6 +33,12 @@ constexpr int kDefaultGroup = 1;  // default cells per tile group along the last
 constexpr int kAnchorBlock = 8;   // group widths are powers of two dividing this
 constexpr int kSparseRuns = 8;    // cell groups with fewer runs are updated one lane per run,
 constexpr int kSparseFactor = 4;  // in iterations with fewer than kSparseFactor runs per cell
+
+// Expands X(D) for every dimension count the kernels are instantiated for;
+// host code dispatches on the runtime d with `switch (d) { EGG_EACH_D(X) }`.
+#define EGG_EACH_D(X)                                                                       \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)  \
+    X(17) X(18) X(19) X(20)
 
 // Whether the cell groups with `runs` runs go to the one-lane-per-run update
 // in this iteration: only when the grid is sparse overall, where streaming a
@@ -58,10 +66,10 @@ struct Geo {
     int bits_w;        // bits of one coordinate field in the key
     int bits_o;        // bits of the outer id field
     int key_bits;      // bits of the cell part of the key
-    int sub_bits;      // bits of the sub-cell part below it (d, or 0 if keys would exceed 128 bits)
+    int sub_bits;      // bits of the sub-cell part below it (d, or 0 if keys would exceed 256 bits)
     int hash_bits;     // bits of the position hash below that (<= 16)
     int sort_bits;     // key_bits + sub_bits + hash_bits
-    int kw;            // 64-bit key words (1 or 2)
+    int kw;            // 64-bit key words (1 .. kMaxKW)
     int wpow[kMaxD];   // w^l, l < od
     int qblocks;       // workgroups of the tile-queue kernels (0 = automatic)
     int qthreads;      // threads per workgroup of those kernels (64..256, multiple of 64)

@@ -62,13 +62,13 @@ void launch_fill(int *p, int count, int v, const DevState *st, hipStream_t s) {
 // least significant last, so numeric key order = lexicographic (outer, g_od,
 // ..., g_{d-1}, sub_0, ..., sub_{d-1}, position hash); the split bits and the
 // hash are present when sub_bits / hash_bits > 0.
-__device__ __forceinline__ void put_bits(uint64_t &lo, uint64_t &hi, uint64_t v, int shift,
+__device__ __forceinline__ void put_bits(uint64_t (&w)[kMaxKW], uint64_t v, int shift,
                                          int width) {
-    if (shift >= 64) {
-        hi |= v << (shift - 64);
-    } else {
-        lo |= v << shift;
-        if (shift + width > 64) hi |= v >> (64 - shift);
+    const int q = shift >> 6, o = shift & 63;
+#pragma unroll
+    for (int k = 0; k < kMaxKW; k++) {
+        if (k == q) w[k] |= v << o;
+        if (k == q + 1 && o + width > 64) w[k] |= v >> (64 - o);  // o > 0 here: width <= 32
     }
 }
 
@@ -85,28 +85,28 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float *__restrict__ X, Ge
             x[l] = X[(long long)l * n + i];
             c[l] = cell_coord(x[l], g.cs, g.w);
         }
-        uint64_t lo = 0, hi = 0;
+        uint64_t kw[kMaxKW] = {};
         int shift = 0;
         if (g.hash_bits) {
             uint32_t h = 0x9E3779B9u;
 #pragma unroll
             for (int l = 0; l < D; l++) h = pos_hash_step(h, __float_as_uint(x[l]));
             h = pos_hash_final(h);
-            put_bits(lo, hi, (uint64_t)(h & ((1u << g.hash_bits) - 1u)), 0, g.hash_bits);
+            put_bits(kw, (uint64_t)(h & ((1u << g.hash_bits) - 1u)), 0, g.hash_bits);
             shift = g.hash_bits;
         }
         if (g.sub_bits) {
             // split bits, sub_{d-1} least significant
 #pragma unroll
             for (int l = D - 1; l >= 0; l--) {
-                put_bits(lo, hi, x[l] >= cell_mid(c[l], g.cs) ? 1u : 0u, shift, 1);
+                put_bits(kw, x[l] >= cell_mid(c[l], g.cs) ? 1u : 0u, shift, 1);
                 shift += 1;
             }
         }
 #pragma unroll
         for (int l = D - 1; l >= 0; l--) {
             if (l >= g.od) {
-                put_bits(lo, hi, (uint64_t)c[l], shift, g.bits_w);
+                put_bits(kw, (uint64_t)c[l], shift, g.bits_w);
                 shift += g.bits_w;
             }
         }
@@ -114,10 +114,30 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float *__restrict__ X, Ge
 #pragma unroll
         for (int l = 0; l < D; l++)
             if (l < g.od) outer += (uint64_t)c[l] * (uint64_t)g.wpow[l];
-        put_bits(lo, hi, outer, shift, g.bits_o);
-        keys[i] = lo;
-        if (g.kw == 2) keys[(long long)n + i] = hi;
+        put_bits(kw, outer, shift, g.bits_o);
+#pragma unroll
+        for (int k = 0; k < kMaxKW; k++)
+            if (k < g.kw) keys[(long long)k * n + i] = kw[k];
     }
+}
+
+// Whether sorted keys i and i-1 agree above their low `s` bits.
+__device__ __forceinline__ bool same_above(const uint64_t *__restrict__ skeys, int n, int i,
+                                           int kw, int s) {
+    const int q = s >> 6, o = s & 63;
+    bool eq = true;
+#pragma unroll
+    for (int k = 0; k < kMaxKW; k++) {
+        if (k < kw && k >= q) {
+            uint64_t a = skeys[(long long)k * n + i], b = skeys[(long long)k * n + i - 1];
+            if (k == q) {
+                a >>= o;
+                b >>= o;
+            }
+            eq = eq && a == b;
+        }
+    }
+    return eq;
 }
 
 template <int D>
@@ -134,17 +154,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(
         perm_s[i] = perm[j];
         int h = 1, hs = 1;
         if (i > 0) {
-            const uint64_t a0 = skeys[i], b0 = skeys[i - 1];
-            const uint64_t a1 = g.kw == 2 ? skeys[(long long)n + i] : 0;
-            const uint64_t b1 = g.kw == 2 ? skeys[(long long)n + i - 1] : 0;
             // the sub-cell part: the key without its low hash_bits bits;
             // the cell part: without the split bits either
-            const uint64_t sa0 = hb ? (a0 >> hb) | (a1 << (64 - hb)) : a0;
-            const uint64_t sb0 = hb ? (b0 >> hb) | (b1 << (64 - hb)) : b0;
-            hs = sa0 != sb0 || (a1 >> hb) != (b1 >> hb);
-            const uint64_t ca0 = sb ? (a0 >> sb) | (a1 << (64 - sb)) : a0;
-            const uint64_t cb0 = sb ? (b0 >> sb) | (b1 << (64 - sb)) : b0;
-            h = ca0 != cb0 || (a1 >> sb) != (b1 >> sb);
+            hs = !same_above(skeys, n, i, g.kw, hb);
+            h = !same_above(skeys, n, i, g.kw, sb);
         }
         head[i] = h;
         if (shead) shead[i] = hs;
@@ -234,8 +247,7 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
 #define KEYS_CASE(DD) \
     case DD: k_keys<DD><<<gb, kBlock, 0, s>>>(b.X, g, b.keys_a, st_loop); break;
     switch (g.d) {
-        KEYS_CASE(1) KEYS_CASE(2) KEYS_CASE(3) KEYS_CASE(4)
-        KEYS_CASE(5) KEYS_CASE(6) KEYS_CASE(7) KEYS_CASE(8)
+        EGG_EACH_D(KEYS_CASE)
     }
 #undef KEYS_CASE
     int where = radix_sort(b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, g.kw, g.sort_bits, b.hist,
@@ -249,8 +261,7 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
                                            st_loop);                                    \
         break;
     switch (g.d) {
-        GATHER_CASE(1) GATHER_CASE(2) GATHER_CASE(3) GATHER_CASE(4)
-        GATHER_CASE(5) GATHER_CASE(6) GATHER_CASE(7) GATHER_CASE(8)
+        EGG_EACH_D(GATHER_CASE)
     }
 #undef GATHER_CASE
     scan_exclusive(b.head, b.hpos, n, nullptr, b.scan_tmp, nullptr, st_loop, s);
@@ -262,8 +273,7 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
                                           b.cell_start, b.cell_g, b.cell_outer, st_loop, st); \
         break;
     switch (g.d) {
-        CELLS_CASE(1) CELLS_CASE(2) CELLS_CASE(3) CELLS_CASE(4)
-        CELLS_CASE(5) CELLS_CASE(6) CELLS_CASE(7) CELLS_CASE(8)
+        EGG_EACH_D(CELLS_CASE)
     }
 #undef CELLS_CASE
     k_outer_ranges<<<gb, kBlock, 0, s>>>(b.cell_outer, st, b.ob, b.oe, st_loop);
@@ -275,8 +285,7 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
                                              b.sub_start, b.sub_g, b.cell_sub, st, st_loop);  \
         break;
         switch (g.d) {
-            SUB_CASE(1) SUB_CASE(2) SUB_CASE(3) SUB_CASE(4)
-            SUB_CASE(5) SUB_CASE(6) SUB_CASE(7) SUB_CASE(8)
+            EGG_EACH_D(SUB_CASE)
         }
 #undef SUB_CASE
     }
@@ -577,8 +586,7 @@ static void group_summaries(const Geo &g, GridBufs &b, const int *starts, const 
                                                     Ts, Tc, b.partial, st_loop);              \
         break;
     switch (g.d) {
-        SUM_CASE(1) SUM_CASE(2) SUM_CASE(3) SUM_CASE(4)
-        SUM_CASE(5) SUM_CASE(6) SUM_CASE(7) SUM_CASE(8)
+        EGG_EACH_D(SUM_CASE)
     }
 #undef SUM_CASE
 }
@@ -592,8 +600,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
 #define RUN_CASE(DD) \
     case DD: k_run_heads<DD><<<gb, kBlock, 0, s>>>(b.Xs, n, b.head, st_loop); break;
     switch (g.d) {
-        RUN_CASE(1) RUN_CASE(2) RUN_CASE(3) RUN_CASE(4)
-        RUN_CASE(5) RUN_CASE(6) RUN_CASE(7) RUN_CASE(8)
+        EGG_EACH_D(RUN_CASE)
     }
 #undef RUN_CASE
     scan_exclusive(b.head, b.hpos, n, nullptr, b.scan_tmp, nullptr, st_loop, s);

@@ -140,8 +140,7 @@ void launch_labels_sync(const Geo &g, GridBufs &b, int *labels, int *invperm, De
                                                   labels, st);                                \
         break;
     switch (g.d) {
-        LS_CASE(1) LS_CASE(2) LS_CASE(3) LS_CASE(4)
-        LS_CASE(5) LS_CASE(6) LS_CASE(7) LS_CASE(8)
+        EGG_EACH_D(LS_CASE)
     }
 #undef LS_CASE
 }

@@ -220,10 +220,15 @@ int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int
         const int shift = p * 8;
         k_rs_hist<<<ntiles, kBlock, 0, s>>>(kin, n, shift, hist, ntiles, st);
         scan_exclusive(hist, hist, 256 * ntiles, nullptr, scan_tmp, nullptr, st, s);
-        if (kw == 1)
-            k_rs_scatter<1><<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, hist, ntiles, st);
-        else
-            k_rs_scatter<2><<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, hist, ntiles, st);
+#define RS_CASE(KW)                                                                          \
+    case KW:                                                                                 \
+        k_rs_scatter<KW><<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, hist,     \
+                                                   ntiles, st);                              \
+        break;
+        switch (kw) {
+            RS_CASE(1) RS_CASE(2) RS_CASE(3) RS_CASE(4)
+        }
+#undef RS_CASE
         where ^= 1;
         uint64_t *tk = kin; kin = kout; kout = tk;
         vin = vout;

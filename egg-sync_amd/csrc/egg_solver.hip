@@ -112,15 +112,15 @@ int compute_geo(int64_t n, int d, float eps, Geo &g) {
     g.bits_w = bitlen((long long)w - 1);
     g.bits_o = bitlen(O - 1);
     g.key_bits = g.bits_o + (d - od) * g.bits_w;
-    if (g.key_bits > 128) return EGG_SYNC_E_KEY;
-    g.sub_bits = g.key_bits + d <= 128 ? d : 0;
+    if (g.key_bits > 64 * kMaxKW) return EGG_SYNC_E_KEY;
+    g.sub_bits = g.key_bits + d <= 64 * kMaxKW ? d : 0;
     {
-        // position hash: up to 16 bits, without growing the key past one word if it fits one
+        // position hash: up to 16 bits, in the room left in the key's last word
         const int used = g.key_bits + g.sub_bits;
-        g.hash_bits = std::min(16, (used <= 64 ? 64 : 128) - used);
+        g.hash_bits = std::min(16, (used + 63) / 64 * 64 - used);
     }
     g.sort_bits = g.key_bits + g.sub_bits + g.hash_bits;
-    g.kw = g.sort_bits > 64 ? 2 : 1;
+    g.kw = std::max(1, (g.sort_bits + 63) / 64);
     return EGG_SYNC_OK;
 }
 
@@ -1147,7 +1147,7 @@ const char *egg_sync_error_string(int code) {
         case EGG_SYNC_E_ARG: return "invalid argument";
         case EGG_SYNC_E_RANGE: return "input values must be finite and lie in [0, 1]";
         case EGG_SYNC_E_DIM: return "dimension above EGG_SYNC_MAX_D";
-        case EGG_SYNC_E_KEY: return "grid too fine: cell keys exceed 128 bits";
+        case EGG_SYNC_E_KEY: return "grid too fine: cell keys exceed 256 bits";
         case EGG_SYNC_E_ALLOC: return "device allocation failed";
         case EGG_SYNC_E_HIP: return "HIP runtime error";
         case EGG_SYNC_E_NODEV: return "no GPU device";

@@ -1130,8 +1130,7 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
                                                             d);                               \
         break;
     switch (g.d) {
-        UPD_CASE(1) UPD_CASE(2) UPD_CASE(3) UPD_CASE(4)
-        UPD_CASE(5) UPD_CASE(6) UPD_CASE(7) UPD_CASE(8)
+        EGG_EACH_D(UPD_CASE)
     }
 #undef UPD_CASE
 }
@@ -1147,8 +1146,7 @@ void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s) 
                                                 b.cell_run, b.pairs, b.pair_cap, g, st);      \
         break;
     switch (g.d) {
-        EX_CASE(1) EX_CASE(2) EX_CASE(3) EX_CASE(4)
-        EX_CASE(5) EX_CASE(6) EX_CASE(7) EX_CASE(8)
+        EGG_EACH_D(EX_CASE)
     }
 #undef EX_CASE
 }
@@ -1169,8 +1167,7 @@ void launch_update_sync(const Geo &g, GridBufs &b, DevState *st, double *rc_part
                 b.ob, b.oe, g, st, rc_part);                                                 \
         break;
     switch (g.d) {
-        SY_CASE(1) SY_CASE(2) SY_CASE(3) SY_CASE(4)
-        SY_CASE(5) SY_CASE(6) SY_CASE(7) SY_CASE(8)
+        EGG_EACH_D(SY_CASE)
     }
 #undef SY_CASE
     k_rc_finish<<<1, kBlock, 0, s>>>(rc_part, kRcBlocks, g.n, st, write_pos);

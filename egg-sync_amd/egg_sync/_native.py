@@ -16,7 +16,7 @@ OK, CAPPED = 0, 1
 E_ARG, E_RANGE, E_DIM, E_KEY, E_ALLOC, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6, -7
 STOP_EGG, STOP_RC = 0, 1
 LABELS_CELLS, LABELS_SYNC = 0, 1
-MAX_D = 8
+MAX_D = 20
 
 EXPORTED = [
     "egg_sync_default_options", "egg_sync_solve", "egg_sync_solve_ex", "egg_sync_solve_device",

@@ -10,6 +10,7 @@ instructions are disabled.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import glob
 import os
 import shutil
@@ -42,10 +43,16 @@ def build_lib(verbose: bool = False) -> str:
     objs = []
     bdir = os.path.join(PKG, "_build")
     os.makedirs(bdir, exist_ok=True)
+    jobs = []
     for src in HIP_SOURCES:
         obj = os.path.join(bdir, src.replace(".hip", ".o"))
-        _run([hipcc, *HIPCC_FLAGS, "-I", INCLUDE, "-c", os.path.join(CSRC, src), "-o", obj])
+        jobs.append([hipcc, *HIPCC_FLAGS, "-I", INCLUDE, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the sources compile independently: one hipcc per source, side by side
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(workers) as ex:
+        for f in [ex.submit(_run, cmd) for cmd in jobs]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([hipcc, *HIPCC_FLAGS, "-shared", "-Wl,-soname,libeggsync.so", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -13,8 +13,7 @@ read the same files with the same columns and delimiters.
 
 prints one JSON record per (value, round): running time (wall, around the
 call, as run_experiment.py:322-325), per-iteration and per-stage times and
-the device bytes held ("space").  Dimensions above 8 are not supported by
-this build (the reference's sweep goes to 20, run_experiment.py:13).
+the device bytes held ("space").
 """
 from __future__ import annotations
 
@@ -28,7 +27,7 @@ import numpy as np
 PARAMS = {"n": 100_000, "d": 2, "std": 5.0, "cl": 5, "eps": 0.05, "rounds": 3}
 SWEEPS = {
     "n": [2000, 4000, 8000, 16000, 32000, 64000, 128000, 256000, 512000, 1024000],
-    "d": [2, 4, 6, 8],
+    "d": [2, 4, 6, 8, 10, 12, 14, 16, 18, 20],
     "std": [1.0, 5.0, 10.0, 15.0, 20.0],
     "cl": [2, 4, 8, 12, 16],
     "eps": [0.0125, 0.025, 0.050, 0.1, 0.2, 0.500],

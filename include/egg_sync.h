@@ -39,7 +39,7 @@ extern "C" {
 #define EGG_SYNC_E_ARG (-1)         /* bad n / d / eps / pointer               */
 #define EGG_SYNC_E_RANGE (-2)       /* input outside [0,1] or not finite       */
 #define EGG_SYNC_E_DIM (-3)         /* d above the supported maximum           */
-#define EGG_SYNC_E_KEY (-4)         /* cell keys would need more than 128 bits */
+#define EGG_SYNC_E_KEY (-4)         /* cell keys would need more than 256 bits */
 #define EGG_SYNC_E_ALLOC (-5)       /* device allocation failed                */
 #define EGG_SYNC_E_HIP (-6)         /* HIP runtime error                       */
 #define EGG_SYNC_E_NODEV (-7)       /* no GPU visible                          */
@@ -49,7 +49,7 @@ extern "C" {
 #define EGG_SYNC_LABELS_CELLS 0     /* EGG_SynC.cu:1205-1227 */
 #define EGG_SYNC_LABELS_SYNC 1      /* synCluster_v2, SynC.cpp:88-129 */
 
-#define EGG_SYNC_MAX_D 8
+#define EGG_SYNC_MAX_D 20
 
 typedef struct egg_sync_options {
     int32_t stop_rule;       /* EGG_SYNC_STOP_*                                   */

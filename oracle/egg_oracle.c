@@ -114,17 +114,18 @@ static int bitlen64(int64_t v) {
     return b < 1 ? 1 : b;
 }
 
-/* The GPU's sort key: cell fields, then d split bits if they fit 128 bits,
- * then up to 16 bits of position hash (not growing a one-word key past 64). */
+/* The GPU's sort key: cell fields, then d split bits if they fit 256 bits,
+ * then up to 16 bits of position hash in the room left in the key's last
+ * 64-bit word. */
 static int key_bits_of(const eo_geom *g) {
     return bitlen64(g->O - 1) + (g->d - g->od) * bitlen64((int64_t)g->w - 1);
 }
 
-static int sub_cells_on(const eo_geom *g) { return key_bits_of(g) + g->d <= 128; }
+static int sub_cells_on(const eo_geom *g) { return key_bits_of(g) + g->d <= 256; }
 
 static int hash_bits_of(const eo_geom *g) {
     int used = key_bits_of(g) + (sub_cells_on(g) ? g->d : 0);
-    int room = (used <= 64 ? 64 : 128) - used;
+    int room = (used + 63) / 64 * 64 - used;
     return room < 16 ? room : 16;
 }
 

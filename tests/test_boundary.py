@@ -81,11 +81,14 @@ def test_errors_are_codes_not_exits():
     out = (ctypes.c_int64 * 5)()
     cs = ctypes.c_float()
     assert L.egg_sync_geometry(10, 0, 0.1, out, ctypes.byref(cs)) == nat.E_ARG
-    assert L.egg_sync_geometry(10, 9, 0.1, out, ctypes.byref(cs)) == nat.E_DIM
+    assert L.egg_sync_geometry(10, 21, 0.1, out, ctypes.byref(cs)) == nat.E_DIM
     assert L.egg_sync_geometry(10, 2, 0.0, out, ctypes.byref(cs)) == nat.E_ARG
     assert L.egg_sync_geometry(10, 2, float("nan"), out, ctypes.byref(cs)) == nat.E_ARG
     assert L.egg_sync_geometry(0, 2, 0.1, out, ctypes.byref(cs)) == nat.E_ARG
-    assert L.egg_sync_geometry(10, 8, 1e-6, out, ctypes.byref(cs)) == nat.E_KEY
+    # d=12, eps=1e-6: 12 coordinate fields of 23 bits = 276 bits > 256
+    assert L.egg_sync_geometry(10, 12, 1e-6, out, ctypes.byref(cs)) == nat.E_KEY
+    assert L.egg_sync_geometry(10, 8, 1e-6, out, ctypes.byref(cs)) == nat.OK
+    assert out[4] == 8 * 23
     assert "[0, 1]" in nat.error_string(nat.E_RANGE)
     assert L.egg_sync_solve_ex(None, 5, 2, 0.1, None, None, None, None) == nat.E_ARG
 

@@ -190,8 +190,8 @@ def test_experiment_harness_records():
         assert r["running_time"] > 0 and r["iterations"] >= 1 and r["space"] > 0
         assert list(r["stage_times"]) == E.STAGES
         assert len(r["itr_times"]) == r["iterations"]
-    recs = E.run("d", [3, 8], rounds=1, n=3000, eps=0.2)
-    assert [r["d"] for r in recs] == [3, 8]
+    recs = E.run("d", [3, 8, 20], rounds=1, n=3000, eps=0.2)
+    assert [r["d"] for r in recs] == [3, 8, 20]
 
 
 def test_batch_config3_slowest_instance_against_oracle(S, O):

@@ -37,11 +37,19 @@ def _cases_grid():
     cases = [("c0", *data.config0()),
              ("c1_20k", data.config1(n=20000)[0], 0.02),
              ("c2_5k", data.config2(n=5000)[0], 0.1)]
-    for d, eps in [(1, 0.01), (4, 0.1), (5, 0.15), (6, 0.2), (7, 0.3), (8, 0.01)]:
+    # d >= 9: keys of 3 and 4 words (d=20 eps=0.05: 192 sort bits, eps=0.02: 216),
+    # and without split bits (d=19 eps=0.002: 247 cell bits + 9 hash bits)
+    for d, eps in [(1, 0.01), (4, 0.1), (5, 0.15), (6, 0.2), (7, 0.3), (8, 0.01), (9, 0.2),
+                   (12, 0.3), (17, 0.4), (20, 0.05), (20, 0.02), (19, 0.002)]:
         X = rng.random((3000, d)).astype(f32)
         X[:3] = 1.0
         X[3:6] = 0.0
-        cases.append((f"rand_d{d}", X, eps))
+        cases.append((f"rand_d{d}_eps{eps}", X, eps))
+    for d, eps in [(12, 0.3), (20, 0.3)]:
+        X = data.gaussian_blobs(3000, d, 5, seed=d, center_lo=0.2, center_hi=0.8, sigma=0.03,
+                                noise=0.05)
+        X[:40] = X[40]   # a run of equal positions
+        cases.append((f"blobs_d{d}", X, eps))
     return cases
 
 
@@ -66,6 +74,10 @@ def _cases_iter():
     cases.append(("c2_4k", data.config2(n=4000)[0], 0.1))
     X0, eps0 = data.config0()
     cases.append(("c0_tight", X0, 0.01))
+    for d, eps in [(12, 0.3), (20, 0.3)]:
+        X = data.gaussian_blobs(3000, d, 5, seed=d, center_lo=0.2, center_hi=0.8, sigma=0.03,
+                                noise=0.05)
+        cases.append((f"blobs_d{d}", X, eps))
     return cases
 
 
@@ -112,7 +124,11 @@ def _cases_solve():
             ("d1", data.gaussian_blobs(2000, 1, 5, seed=9, center_lo=0.1, center_hi=0.9,
                                        sigma=0.02, noise=0.05), 0.02),
             ("d5", data.gaussian_blobs(3000, 5, 6, seed=11, center_lo=0.2, center_hi=0.8,
-                                       sigma=0.05, noise=0.05), 0.15)]
+                                       sigma=0.05, noise=0.05), 0.15),
+            ("d12", data.gaussian_blobs(3000, 12, 5, seed=12, center_lo=0.2, center_hi=0.8,
+                                        sigma=0.03, noise=0.05), 0.3),
+            ("d20", data.gaussian_blobs(3000, 20, 5, seed=20, center_lo=0.2, center_hi=0.8,
+                                        sigma=0.03, noise=0.05), 0.3)]
 
 
 @pytest.mark.parametrize("case", _cases_solve(), ids=lambda c: c[0])
@@ -183,7 +199,7 @@ def test_edge_cases(S, O):
             S.solve(bad, 0.05)
         assert e.value.code == nat.E_RANGE
     with pytest.raises(nat.EggSyncError):
-        S.solve(np.zeros((4, 9), f32), 0.05)
+        S.solve(np.zeros((4, 21), f32), 0.05)
 
 
 def test_deterministic(S):

@@ -96,11 +96,24 @@ def test_sqrt_threshold(O, e):
 
 # ---------------------------------------------------------- grid build ----
 
+def _bitlen(v):
+    return max(1, int(v).bit_length())
+
+
 @pytest.mark.parametrize("d,n,eps,seed", [(1, 500, 0.05, 0), (2, 2000, 0.05, 1), (3, 3000, 0.05, 2),
-                                          (5, 1500, 0.2, 3), (8, 1000, 0.3, 4)])
+                                          (5, 1500, 0.2, 3), (8, 1000, 0.3, 4), (12, 1500, 0.3, 5),
+                                          (20, 1500, 0.3, 6), (19, 1000, 0.002, 7)])
 def test_grid_invariants(O, d, n, eps, seed):
     rng = np.random.default_rng(seed)
-    X = rng.random((n, d)).astype(f32)
+    if d <= 8:
+        X = rng.random((n, d)).astype(f32)
+    else:
+        # one tight cluster around the centre of a cell, so that cells hold
+        # many points and their order inside a cell (all d split bits, hash)
+        # is exercised
+        cs = O.geometry(n, d, eps).cs
+        c = (np.floor(0.5 / cs) + 0.5) * cs
+        X = np.clip(c + cs / 8 * rng.standard_normal((n, d)), 0, 1).astype(f32)
     X[:5] = 1.0  # exercise the w -> w-1 clamp
     X[5:10] = 0.0
     X[10:20:2] = X[11]  # repeated positions, interleaved with others
@@ -119,14 +132,22 @@ def test_grid_invariants(O, d, n, eps, seed):
     keys = [tuple([o] + list(r[g.od:])) for o, r in zip(outer, cg)]
     assert keys == sorted(keys) and len(set(keys)) == len(keys)
     # inside a cell, points are ordered by sub-cell (split bits at the cell
-    # centre, computed in FP32, dimension 0 first), then by the low 16 bits of
-    # a hash of the position's bit patterns, and stably within equal keys;
-    # bitwise-equal positions therefore sit next to each other
+    # centre, computed in FP32, dimension 0 first; present while the key
+    # stays within 256 bits), then by the low bits of a hash of the
+    # position's bit patterns (up to 16, as many as the key's last 64-bit
+    # word has room for), and stably within equal keys; bitwise-equal
+    # positions therefore sit next to each other
+    key_bits = _bitlen(g.O - 1) + (d - g.od) * _bitlen(g.w - 1)
+    sub_on = key_bits + d <= 256
+    used = key_bits + (d if sub_on else 0)
+    hash_bits = min(16, -(-used // 64) * 64 - used)
+    if d == 19:
+        assert not sub_on and hash_bits == 9
     mid = ((coords.astype(f32) + f32(0.5)) * f32(g.cs)).astype(f32)
     sub = (X >= mid).astype(np.int64)
     sub_key = np.zeros(n, np.int64)
     for l in range(d):
-        sub_key = sub_key * 2 + sub[:, l]
+        sub_key = sub_key * 2 + (sub[:, l] if sub_on else 0)
     h = np.full(n, 0x9E3779B9, np.uint64)
     bits = X.view(np.uint32).astype(np.uint64)
     for l in range(d):
@@ -135,7 +156,7 @@ def test_grid_invariants(O, d, n, eps, seed):
         h ^= h >> 13
     h = (h * 0xC2B2AE35) & 0xFFFFFFFF
     h ^= h >> 16
-    full_key = sub_key * 65536 + (h & 0xFFFF).astype(np.int64)
+    full_key = sub_key * (1 << hash_bits) + (h & ((1 << hash_bits) - 1)).astype(np.int64)
     rank = np.empty(n, np.int64)
     rank[order_in] = np.arange(n)
     for c in range(G["C"]):
