@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/egg_sync.h"
+#include "egg_host.h"
 #include "egg_internal.h"
 
 namespace egg {
@@ -55,18 +56,6 @@ int bitlen(long long v) {
         v >>= 1;
     }
     return b < 1 ? 1 : b;
-}
-
-float sqrt_threshold(float e) {
-    float s = e * e;
-    if (std::isinf(s)) s = 3.4028234663852886e38f;
-    while (s > 0.0f && std::sqrt(s) > e) s = std::nextafter(s, 0.0f);
-    for (;;) {
-        float nx = std::nextafter(s, INFINITY);
-        if (std::isinf(nx) || std::sqrt(nx) > e) break;
-        s = nx;
-    }
-    return s;
 }
 
 // Geometry exactly as EGG_SynC.cu:1249-1260, :563, :779, :1045, :1430.
@@ -211,13 +200,6 @@ Layout carve(void *base, const Geo &g) {
     L.err = c.take<int>(1);
     L.bytes = c.off;
     return L;
-}
-
-int hip_err(hipError_t e) {
-    if (e == hipSuccess) return 0;
-    if (e == hipErrorOutOfMemory) return EGG_SYNC_E_ALLOC;
-    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return EGG_SYNC_E_NODEV;
-    return EGG_SYNC_E_HIP;
 }
 
 #define HIP_TRY(x)                       \

@@ -5,6 +5,8 @@
 //   EGG_SynC_wrapper(D, eps)                      map.cpp:31-33 -> EGG_SynC.cu:1229
 //   DynamicalClustering_wrapper(D, eps, lam)      map.cpp:16-18 -> SynC.cpp:268 (run on the GPU)
 //   DynamicalClustering_parallel_wrapper(...)     map.cpp:20-22 -> SynC.cpp:328 (same GPU path)
+//   GPU_DynamicalClustering_wrapper(D, eps, lam)  map.cpp:27-29 -> simple_GPU_SynC.cu:105
+//                                                 (brute force, labels float32)
 // Each returns [labels, itr_times, stage_times, space] like the reference
 // (EGG_SynC.cu:1587-1600, SynC.cpp:309-325): labels float32 for EGG-SynC and
 // int32 for SynC, itr_times float32 seconds per iteration, stage_times
@@ -82,6 +84,41 @@ std::vector<at::Tensor> DynamicalClustering_wrapper(at::Tensor &D, float eps, fl
     return run(D, eps, EGG_SYNC_STOP_RC, EGG_SYNC_LABELS_SYNC, lam, false, stream, 0);
 }
 
+std::vector<at::Tensor> GPU_DynamicalClustering_wrapper(at::Tensor &D, float eps, float lam,
+                                                        int64_t stream) {
+    check_input(D);
+    D = D.contiguous();
+    const int64_t n = D.size(0);
+    const int32_t d = (int32_t)D.size(1);
+    std::vector<float> itr(4096, 0.f);
+    egg_sync_stats st{};
+    st.itr_ms = itr.data();
+    st.itr_ms_cap = (int32_t)itr.size();
+    at::Tensor labels;
+    int rc;
+    if (D.is_cuda()) {
+        labels = at::empty({n}, D.options().dtype(at::kInt));
+        pybind11::gil_scoped_release nogil;
+        rc = egg_sync_simple_solve_device(D.data_ptr<float>(), n, d, eps, lam, 0,
+                                          labels.data_ptr<int>(), nullptr, &st,
+                                          reinterpret_cast<void *>(stream));
+    } else {
+        labels = at::empty({n}, at::kInt);
+        pybind11::gil_scoped_release nogil;
+        rc = egg_sync_simple_solve(D.data_ptr<float>(), n, d, eps, lam, 0, labels.data_ptr<int>(),
+                                   nullptr, &st);
+    }
+    raise_if_error(rc);
+    const int iters = st.iterations;
+    at::Tensor itr_times = at::empty({iters}, at::kFloat);
+    for (int i = 0; i < iters; i++)
+        itr_times[i] = i < st.itr_ms_cap ? itr[i] / 1000.f : 0.f;
+    at::Tensor stage_times = at::empty({6}, at::kFloat);
+    for (int k = 0; k < 6; k++) stage_times[k] = (float)(st.stage_ms[k] / 1000.0);
+    at::Tensor space = at::full({1}, st.workspace_bytes, at::kLong);
+    return {labels.to(at::kFloat), itr_times, stage_times, space};
+}
+
 }  // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
@@ -89,6 +126,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
           pybind11::arg("D"), pybind11::arg("eps"), pybind11::arg("stream") = 0);
     m.def("DynamicalClustering_wrapper", &DynamicalClustering_wrapper,
           "SynC solve, r_c stop (SynC.cpp:268), on the GPU", pybind11::arg("D"),
+          pybind11::arg("eps"), pybind11::arg("lam"), pybind11::arg("stream") = 0);
+    m.def("GPU_DynamicalClustering_wrapper", &GPU_DynamicalClustering_wrapper,
+          "brute-force SynC on the GPU (simple_GPU_SynC.cu:105)", pybind11::arg("D"),
           pybind11::arg("eps"), pybind11::arg("lam"), pybind11::arg("stream") = 0);
     m.def("DynamicalClustering_parallel_wrapper", &DynamicalClustering_wrapper,
           "SynC solve, r_c stop (SynC.cpp:328), on the GPU", pybind11::arg("D"),

@@ -24,7 +24,8 @@ EXPORTED = [
     "egg_sync_geometry", "egg_sync_release", "egg_sync_error_string",
     "egg_sync_features_device", "egg_sync_jump_device", "egg_mlp_param_count",
     "egg_mlp_workspace_floats", "egg_mlp_forward_device", "egg_mlp_train_step_device",
-    "egg_sync_solve_batch_device",
+    "egg_sync_solve_batch_device", "egg_sync_simple_solve_device", "egg_sync_simple_solve",
+    "egg_sync_simple_labels_device",
 ]
 
 
@@ -105,6 +106,12 @@ def lib():
                                                   ctypes.POINTER(Options), P, P,
                                                   ctypes.POINTER(InstanceResult),
                                                   ctypes.POINTER(Stats), P]
+        L.egg_sync_simple_solve_device.argtypes = [P, i64, i32, f32, f32, i32, P, P,
+                                                   ctypes.POINTER(Stats), P]
+        L.egg_sync_simple_solve.argtypes = [P, i64, i32, f32, f32, i32, P, P,
+                                            ctypes.POINTER(Stats)]
+        L.egg_sync_simple_labels_device.argtypes = [P, i64, i32, f32, P,
+                                                    ctypes.POINTER(ctypes.c_int32), P]
         L.egg_sync_error_string.argtypes = [ctypes.c_int]
         L.egg_sync_error_string.restype = ctypes.c_char_p
         _lib = L

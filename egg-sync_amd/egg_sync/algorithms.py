@@ -5,6 +5,8 @@
     SynC_parallel(D, eps)       python/algorithms.py:94-95
     DynamicalClustering(D, eps, lam=1 - 1e-3, call=None)            :21-22
     DynamicalClustering_parallel(D, eps, lam=1 - 1e-3, call=None)   :25-26
+    simple_GPU_SynC(D, eps)     python/algorithms.py:106-107 (brute force, simple_GPU_SynC.cu)
+    GPU_DynamicalClustering(D, eps, lam=1 - 1e-3, call=None)        :33-34
 
 Each returns ``[labels, itr_times, stage_times, space]`` like the reference
 (EGG_SynC.cu:1587-1600, SynC.cpp:309-325).  ``D`` is an [n, d] float32 tensor
@@ -68,3 +70,13 @@ def SynC(D: torch.Tensor, eps: float):
 
 def SynC_parallel(D: torch.Tensor, eps: float):
     return DynamicalClustering_parallel(D, eps, lam=1 - 1e-3)
+
+
+def GPU_DynamicalClustering(D: torch.Tensor, eps: float, lam: float = 1 - 1e-3, call=None):
+    """Brute-force SynC on the GPU: r_c >= lam stop with the reference's 100-pass cap,
+    labels float32 (smallest eps-neighbour index, numbered 0..k-1; -1 = noise)."""
+    return _ext().GPU_DynamicalClustering_wrapper(D, float(eps), float(lam), _stream_of(D))
+
+
+def simple_GPU_SynC(D: torch.Tensor, eps: float):
+    return GPU_DynamicalClustering(D, eps)

@@ -25,7 +25,7 @@ INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libeggsync.so")
 EXT = os.path.join(PKG, "_egg_sync_C.so")
 HIP_SOURCES = ["egg_scan_sort.hip", "egg_grid.hip", "egg_update.hip", "egg_labels.hip",
-               "egg_solver.hip", "egg_mlp.hip"]
+               "egg_solver.hip", "egg_mlp.hip", "egg_brute.hip"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
                "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

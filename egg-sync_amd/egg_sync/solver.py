@@ -201,6 +201,47 @@ def iteration(X, eps: float, order_in=None):
                 synced=synced.cpu().numpy(), flag_local=fl.value, flag_final=ff.value)
 
 
+def simple_solve(X, eps: float, *, lam=1 - 1e-3, max_iter=100, itr_cap=4096):
+    """Brute-force SynC on the GPU (simple_GPU_SynC.cu:105-237): every point scans
+    all points each pass; stops once r >= lam or after max_iter passes (the
+    reference's cap is 100).  Returns dict(labels int32 numpy -- smallest index
+    among a point's eps-neighbours, numbered 0..k-1 --, X final numpy, stats)."""
+    D = _dev_f32(X)
+    n, d = D.shape
+    labels = torch.empty(n, dtype=torch.int32, device=D.device)
+    final = torch.empty_like(D)
+    st = nat.Stats()
+    itr = np.zeros(itr_cap, np.float32)
+    st.itr_ms = itr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    st.itr_ms_cap = itr_cap
+    r_tr = np.zeros(itr_cap, np.float64)
+    st.trace_cap = itr_cap
+    st.rc_trace = r_tr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rc = nat.lib().egg_sync_simple_solve_device(nat.ptr(D), n, d, float(eps), float(lam),
+                                                int(max_iter), nat.ptr(labels), nat.ptr(final),
+                                                ctypes.byref(st), _stream())
+    nat.check(rc, "egg_sync_simple_solve_device")
+    it = st.iterations
+    stats = {k: getattr(st, k) for k in ("iterations", "status", "n_clusters", "rc", "total_ms",
+                                         "workspace_bytes")}
+    stats["stage_ms"] = list(st.stage_ms)
+    stats["itr_ms"] = itr[:min(it, itr_cap)].copy()
+    stats["r_trace"] = r_tr[:min(it, itr_cap)].astype(np.float32)
+    return dict(labels=labels.cpu().numpy(), X=final.cpu().numpy(), stats=stats, code=rc)
+
+
+def simple_labels(X, eps: float):
+    """Labels of GPU_synCluster + rename_cluster (simple_GPU_SynC.cu:70-102) on the GPU."""
+    D = _dev_f32(X)
+    n, d = D.shape
+    labels = torch.empty(n, dtype=torch.int32, device=D.device)
+    k = ctypes.c_int32()
+    nat.check(nat.lib().egg_sync_simple_labels_device(nat.ptr(D), n, d, float(eps),
+                                                      nat.ptr(labels), ctypes.byref(k),
+                                                      _stream()), "egg_sync_simple_labels_device")
+    return labels.cpu().numpy(), k.value
+
+
 def rc(X, eps: float) -> float:
     D = _dev_f32(X)
     out = ctypes.c_double()

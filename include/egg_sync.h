@@ -20,6 +20,11 @@
  *   egg_sync_rc_device           order parameter r_c (SynC.cpp:42-56, FP64 sums)
  *   egg_sync_solve_batch_device  many EGG_SynC calls at once (run_experiment.py:294-331 loops
  *                                over datasets and eps values one call at a time)
+ *   egg_sync_simple_solve[_device]  std::vector<at::Tensor> GPU_DynamicalClustering(float *h_D,
+ *                            int n, int d, float eps, float lam) (src/algorithms/simple_GPU_SynC.cuh:11,
+ *                            simple_GPU_SynC.cu:105), reached through GPU_DynamicalClustering_wrapper
+ *                            (src/maps/map.cpp:27-29) and simple_GPU_SynC (python/algorithms.py:106-107)
+ *   egg_sync_simple_labels_device   GPU_synCluster + rename_cluster (simple_GPU_SynC.cu:70-102)
  * Threading: one solve at a time per process (the library caches one
  * workspace per device, like the reference's global allocation map,
  * GPU_utils.cu:176-178); calls are synchronous with respect to the host.
@@ -140,6 +145,31 @@ int egg_sync_solve_batch_device(const float *d_D, const egg_sync_instance *inst,
                                 int32_t *d_labels, float *d_final,
                                 egg_sync_instance_result *results, egg_sync_stats *stats,
                                 void *stream);
+
+/* ---- Brute-force SynC (simple_GPU_SynC.cu): every point scans all n points,
+ * O(n^2) per pass.  x'_p = x_p + mean_{|x_q - x_p| <= eps} sin(x_q - x_p), FP32
+ * sums in ascending q; r = mean_p mean_q exp(-|x_q - x_p|) (FP64 mean of the
+ * FP32 per-point values); passes stop once r >= lam or after max_iter passes
+ * (max_iter <= 0: 100, the reference's cap, simple_GPU_SynC.cu:126; returns
+ * EGG_SYNC_CAPPED when the cap stopped it).  Labels: a point with another
+ * point within eps gets the smallest index among its eps-neighbours, and
+ * those representatives are numbered 0..k-1 in index order; others -1.
+ * Positions are row-major n x d with any finite values (no grid, no [0,1]
+ * requirement).  stats (may be NULL): iterations, status, n_clusters, rc = r
+ * of the last pass, stage_ms 0/2/4/5, itr_ms / rc_trace if given,
+ * workspace_bytes; n_cells is 0. */
+int egg_sync_simple_solve_device(const float *d_D, int64_t n, int32_t d, float eps, float lam,
+                                 int32_t max_iter, int32_t *d_labels, float *d_final,
+                                 egg_sync_stats *stats, void *stream);
+
+/* Host arrays in and out (final_out, n x d, may be NULL). */
+int egg_sync_simple_solve(const float *h_D, int64_t n, int32_t d, float eps, float lam,
+                          int32_t max_iter, int32_t *labels_out, float *final_out,
+                          egg_sync_stats *stats_out);
+
+/* The labels above for device positions d_D; *n_clusters (host, may be NULL). */
+int egg_sync_simple_labels_device(const float *d_D, int64_t n, int32_t d, float eps,
+                                  int32_t *d_labels, int32_t *n_clusters, void *stream);
 
 /* Grid build on device positions d_D (n x d) visited in order d_order_in
  * (point ids; NULL = identity).  Outputs (device, may be NULL):

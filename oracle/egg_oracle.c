@@ -332,6 +332,41 @@ int eo_sync_labels_f32(const float *X, int64_t n, int32_t d, float eps, int32_t 
     return cl;
 }
 
+/* GPU_synCluster + rename_cluster (simple_GPU_SynC.cu:70-102): a point with
+ * more than one eps-neighbour (itself included; FP32 distance of
+ * CPU_math.cpp:7-14 / simple_GPU_SynC.cu:76-82) gets the smallest index among
+ * them; labels are those representatives numbered 0..k-1 in index order
+ * (inclusive scan of the representative flags, minus one). */
+int eo_simple_labels_f32(const float *X, int64_t n, int32_t d, float eps, int32_t *C) {
+    uint8_t *incl = (uint8_t *)calloc((size_t)(n > 0 ? n : 1), 1);
+    int32_t *map = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    if (!incl || !map) { free(incl); free(map); return -2; }
+    for (int64_t p = 0; p < n; p++) {
+        int64_t mn = p, count = 0;
+        for (int64_t q = 0; q < n; q++) {
+            if (distance_f32(&X[p * d], &X[q * d], d) <= eps) {
+                count++;
+                if (q < mn) mn = q;
+            }
+        }
+        C[p] = -1;
+        if (count > 1) {
+            C[p] = (int32_t)mn;
+            incl[mn] = 1;
+        }
+    }
+    int32_t run = 0;
+    for (int64_t i = 0; i < n; i++) {
+        run += incl[i];
+        map[i] = run;
+    }
+    for (int64_t p = 0; p < n; p++)
+        if (C[p] >= 0) C[p] = map[C[p]] - 1;
+    free(incl);
+    free(map);
+    return run;
+}
+
 /* DynamicalClustering (SynC.cpp:268-326). */
 int eo_sync_solve_f32(const float *X0, int64_t n, int32_t d, float eps, float lam,
                       int32_t max_iter, int32_t *labels, float *X_final, int32_t *iters,

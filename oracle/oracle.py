@@ -51,6 +51,7 @@ def lib():
         L.eo_egg_solve_f64.argtypes = [P, i64, i32, f64, i32, P, P, P]
         L.eo_egg_labels_f32.argtypes = [P, i64, i32, f32, P]
         L.eo_sync_solve_f32.argtypes = [P, i64, i32, f32, f32, i32, P, P, P, P]
+        L.eo_simple_labels_f32.argtypes = [P, i64, i32, f32, P]
         L.eo_sync_iteration_range_f32.argtypes = [P, i64, i32, f32, i64, i64, P, i32]
         L.eo_sync_iteration_range_f32.restype = f64
         L.eo_sync_labels_f32.argtypes = [P, i64, i32, f32, P]
@@ -177,6 +178,28 @@ def sync_labels(X, eps):
     labels = np.empty(X.shape[0], np.int32)
     lib().eo_sync_labels_f32(_p(X), X.shape[0], X.shape[1], eps, _p(labels))
     return labels
+
+
+def simple_labels(X, eps):
+    """Labels of the brute-force GPU SynC (simple_GPU_SynC.cu:70-102); returns (labels, k)."""
+    X = _f32(X)
+    labels = np.empty(X.shape[0], np.int32)
+    k = lib().eo_simple_labels_f32(_p(X), X.shape[0], X.shape[1], eps, _p(labels))
+    if k < 0:
+        raise RuntimeError("simple labels failed")
+    return labels, k
+
+
+def simple_solve(X, eps, lam=np.float32(1 - 1e-3), max_iter=100):
+    """GPU_DynamicalClustering (simple_GPU_SynC.cu:105-237): the SynC pass of
+    SynC.cpp:16-56 (kernel_itr, :29-67, adds the same terms in the same order),
+    r >= lam stop, at most max_iter passes (the reference: 100), then
+    simple_labels.  The reference sums r with float atomics (order not fixed);
+    here the FP32 sum of sync_solve."""
+    r = sync_solve(X, eps, lam=lam, max_iter=max_iter)
+    labels, k = simple_labels(r["X"], eps)
+    r["labels"], r["n_clusters"] = labels, k
+    return r
 
 
 def rc(X, eps) -> float:

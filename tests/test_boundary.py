@@ -97,10 +97,10 @@ def test_torch_extension_exposes_reference_names():
     from egg_sync import algorithms
     ext = algorithms._ext()
     for name in ("EGG_SynC_wrapper", "DynamicalClustering_wrapper",
-                 "DynamicalClustering_parallel_wrapper"):
+                 "DynamicalClustering_parallel_wrapper", "GPU_DynamicalClustering_wrapper"):
         assert hasattr(ext, name)
     for name in ("EGG_SynC", "SynC", "SynC_parallel", "DynamicalClustering",
-                 "DynamicalClustering_parallel"):
+                 "DynamicalClustering_parallel", "GPU_DynamicalClustering", "simple_GPU_SynC"):
         assert callable(getattr(algorithms, name))
 
 

@@ -65,6 +65,61 @@ def test_sync_labels_reference_semantics(O):
     assert list(lab) == [0, 1, 1, -1]
 
 
+def test_simple_labels_known_answers(O):
+    # GPU_synCluster (simple_GPU_SynC.cu:70-95): smallest eps-neighbour index,
+    # not transitive; rename_cluster numbers the representatives in index order
+    X = np.array([[0.0], [0.01], [0.5], [0.51], [0.9]], f32)
+    lab, k = O.simple_labels(X, 0.05)
+    assert list(lab) == [0, 0, 1, 1, -1] and k == 2
+    chain = np.array([[0.0], [0.04], [0.08]], f32)
+    lab, k = O.simple_labels(chain, 0.05)
+    assert list(lab) == [0, 0, 1] and k == 2      # point 2's smallest neighbour is 1
+    lab, k = O.simple_labels(np.full((4, 3), 0.3, f32), 0.01)
+    assert list(lab) == [0, 0, 0, 0] and k == 1
+    lab, k = O.simple_labels(np.array([[0.2, 0.2]], f32), 0.1)
+    assert list(lab) == [-1] and k == 0
+
+
+def _simple_labels_numpy(X, eps):
+    """The same rule, vectorised: FP32 squared distances summed dimension by
+    dimension, sqrt rounded to FP32."""
+    X = X.astype(f32)
+    s = np.zeros((len(X), len(X)), f32)
+    for l in range(X.shape[1]):
+        df = X[:, None, l] - X[None, :, l]
+        s = s + df * df
+    nb = np.sqrt(s) <= f32(eps)
+    cnt = nb.sum(1)
+    first = nb.argmax(1)
+    reps = np.zeros(len(X), bool)
+    reps[first[cnt > 1]] = True
+    rank = np.cumsum(reps) - 1
+    return np.where(cnt > 1, rank[first], -1).astype(np.int32)
+
+
+@pytest.mark.parametrize("d,eps", [(1, 0.02), (2, 0.05), (3, 0.1), (8, 0.3), (20, 0.5)])
+def test_simple_labels_against_numpy(O, d, eps):
+    rng = np.random.default_rng(d)
+    X = rng.random((400, d)).astype(f32)
+    X[50:60] = X[60]
+    lab, k = O.simple_labels(X, eps)
+    ref = _simple_labels_numpy(X, eps)
+    assert np.array_equal(lab, ref) and k == ref.max() + 1
+
+
+def test_simple_solve_is_the_sync_pass_with_a_cap(O):
+    # kernel_itr (simple_GPU_SynC.cu:29-67) adds the terms of SynC.cpp:16-56 in
+    # the same order, so its passes are the compiled reference's (c0_sync)
+    g = golden("c0_sync")
+    r = O.simple_solve(g["X"], float(g["eps"]), g["lam"])
+    it = int(g["iterations"])
+    assert r["iterations"] == min(it, 100)
+    if it <= 100:
+        assert np.array_equal(r["X"], g["X_final"])
+    r3 = O.simple_solve(g["X"], float(g["eps"]), lam=np.float32(1.5), max_iter=3)
+    assert r3["iterations"] == 3 and r3["status"] == 1
+
+
 # ------------------------------------------------------------ geometry ----
 
 @pytest.mark.parametrize("n,d,eps,cs,w,od,O_,R", [
