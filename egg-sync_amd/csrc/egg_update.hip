@@ -249,17 +249,20 @@ __device__ __forceinline__ unsigned wave_or(unsigned v) {
 }
 
 // Classifies sub-cell sc for a lane at x: 1 if every point of it is within
-// eps (its FP32 squared distance <= thr), 0 if none is, 2 otherwise.  Exact
-// FP64 distances to the sub-cell box, padded by kSubPad against the rounding
-// of cell boundaries, compared with bounds that absorb the FP32 rounding of
-// the reference's distance.
-constexpr double kSubPad = 1e-6;
+// eps (its FP32 squared distance <= thr), 0 if none is, 2 otherwise.  FP32
+// squared distances to the sub-cell box, padded by kSubPad against the
+// rounding of cell boundaries, compared with bounds thr * (1 -+ 64 * 2^-24):
+// this distance and the reference's FP32 distance of any point in the box
+// are each within (d + 2) * 2^-24 relative of their exact values, so for
+// d <= 20 the margin of 64 * 2^-24 covers both.
+constexpr float kSubPad = 1e-6f;
 
 template <int D>
 __device__ __forceinline__ int sub_class(const int *__restrict__ sub_g, int n, int sc,
-                                         const float *x, const Geo &g, double thr_in,
-                                         double thr_out) {
-    double mn = 0.0, mx = 0.0;
+                                         const float *x, const Geo &g, float thr_in,
+                                         float thr_out) {
+    static_assert(2 * D + 4 < 64, "classification margin");
+    float mn = 0.f, mx = 0.f;
 #pragma unroll
     for (int l = 0; l < D; l++) {
         const int f = sub_g[(long long)l * n + sc];
@@ -268,11 +271,11 @@ __device__ __forceinline__ int sub_class(const int *__restrict__ sub_g, int n, i
                                         : (float)(gl + 1) * g.cs;
         const float lo = (f & 1) ? cell_mid(gl, g.cs) : (float)gl * g.cs;
         const float hi = (f & 1) ? top : cell_mid(gl, g.cs);
-        const double dlo = (double)lo - kSubPad, dhi = (double)hi + kSubPad;
-        const double xd = (double)x[l];
-        const double a = (dlo - xd) * (dlo - xd), b = (dhi - xd) * (dhi - xd);
-        mx += a > b ? a : b;
-        mn += xd < dlo ? a : (xd > dhi ? b : 0.0);
+        const float dlo = lo - kSubPad, dhi = hi + kSubPad;
+        const float ta = dlo - x[l], tb = dhi - x[l];
+        const float a = ta * ta, b = tb * tb;
+        mx += fmaxf(a, b);
+        mn += x[l] < dlo ? a : (x[l] > dhi ? b : 0.f);
     }
     if (mx <= thr_in) return 1;
     if (mn >= thr_out) return 0;
@@ -308,11 +311,10 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     const int n = g.n, R = g.R, od = g.od;
     const int G = st->G, T = st->T;
     const float thr = g.thr_eps, thr_half = g.thr_half;
-    // sub-cell classification bounds: the reference's FP32 squared distance
-    // is within (d+1) * 2^-24 relative of the exact one, so an exact bound
-    // below thr_in (above thr_out) decides every point of the sub-cell
-    const double thr_in = (double)thr * (1.0 - 64.0 * 0x1p-24);
-    const double thr_out = (double)thr * (1.0 + 64.0 * 0x1p-24);
+    // sub-cell classification bounds (sub_class); their own rounding keeps a
+    // margin of at least 63 * 2^-24
+    const float thr_in = thr * (1.0f - 64.0f * 0x1p-24f);
+    const float thr_out = thr * (1.0f + 64.0f * 0x1p-24f);
     const int lane = __lane_id();
     SrcPt<D> *src = s_src[threadIdx.x / kWave];
     int *len = s_len[threadIdx.x / kWave];
