@@ -33,6 +33,10 @@ constexpr int kDefaultGroup = 1;  // default cells per tile group along the last
 constexpr int kAnchorBlock = 8;   // group widths are powers of two dividing this
 constexpr int kSparseRuns = 8;    // cell groups with fewer runs are updated one lane per run,
 constexpr int kSparseFactor = 4;  // in iterations with fewer than kSparseFactor runs per cell
+constexpr int kHeavyRuns = 16384; // a streamed range of at least this many runs is heavy:
+constexpr int kUnitRuns = 1024;   // split into units of this many runs, any wavefront's work,
+constexpr int kHeavyCap = 65536;  // with up to this many ranges per pass in the queue,
+constexpr int kHeavyMaxD = 8;     // for d up to this (per-run fixed-point sums: (2d+1)*8 B)
 
 // Expands X(D) for every dimension count the kernels are instantiated for;
 // host code dispatches on the runtime d with `switch (d) { EGG_EACH_D(X) }`.
@@ -85,6 +89,18 @@ __host__ __device__ __forceinline__ float cell_mid(int g, float cs) {
     return ((float)g + 0.5f) * cs;
 }
 
+// Order-preserving map of a float (not NaN) to an unsigned key and back:
+// a < b  <=>  ord_key(a) < ord_key(b), so min / max of positions can be
+// taken with integer atomics.
+__device__ __forceinline__ unsigned ord_key(float v) {
+    const unsigned u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float ord_val(unsigned k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
 // Hash of a position's FP32 bit patterns (the lowest field of the sort key):
 // bitwise-equal positions get equal hashes, so inside a sub-cell they sort
 // next to each other and form one run.
@@ -124,6 +140,12 @@ struct DevState {
     int sparse_count;  // runs of sparse cell groups (k_update_sparse)
     int WS;            // sub-cell summary work items
     double rc;         // order parameter of the last r_c evaluation
+    // heavy streamed ranges of the update (k_update_heavy): descriptors
+    // created (high 32 bits) and work units allocated (low 32 bits), the
+    // work-queue head over those units, tiles whose result waits for them
+    unsigned long long heavy_pack;
+    int unit_next;
+    int ndt;
 };
 
 // ---------------------------------------------------------------- helpers --

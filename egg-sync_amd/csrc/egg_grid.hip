@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_subcells(
     const float *__restrict__ Xs, const int *__restrict__ head, const int *__restrict__ shead,
     const int *__restrict__ spos, const int *__restrict__ point_cell, Geo g,
     int *__restrict__ sub_start, int *__restrict__ sub_g, int *__restrict__ cell_sub,
-    DevState *st, DevState *st_loop) {
+    unsigned *__restrict__ smin, unsigned *__restrict__ smax, DevState *st, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int n = g.n;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -183,6 +183,9 @@ __global__ __launch_bounds__(kBlock) void k_subcells(
                 const float x = Xs[(long long)l * n + i];
                 const int gl = cell_coord(x, g.cs, g.w);
                 sub_g[(long long)l * n + s] = 2 * gl + (x >= cell_mid(gl, g.cs) ? 1 : 0);
+                // empty box, widened by the summary pass (k_summary)
+                smin[(long long)l * n + s] = 0xFFFFFFFFu;
+                smax[(long long)l * n + s] = 0u;
             }
             if (head[i]) cell_sub[point_cell[i]] = s;
         }
@@ -282,7 +285,8 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
 #define SUB_CASE(DD)                                                                          \
     case DD:                                                                                  \
         k_subcells<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.head, b.shead, b.spos, b.point_cell, g,  \
-                                             b.sub_start, b.sub_g, b.cell_sub, st, st_loop);  \
+                                             b.sub_start, b.sub_g, b.cell_sub, b.smin,        \
+                                             b.smax, st, st_loop);                            \
         break;
         switch (g.d) {
             EGG_EACH_D(SUB_CASE)
@@ -414,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_summary(
     const int *__restrict__ cell_g, const int *__restrict__ wi_off,
     const int *__restrict__ nchunk, Geo g, const int *count, const int *nwork,
     float *__restrict__ Ts, float *__restrict__ Tc, float *__restrict__ partial,
-    DevState *st_loop) {
+    unsigned *__restrict__ mnk, unsigned *__restrict__ mxk, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int n = g.n, C = *count, W = *nwork;
     const int lane = __lane_id();
@@ -437,8 +441,11 @@ __global__ __launch_bounds__(kBlock) void k_summary(
             const float a = (float)cell_g[(long long)l * n + c] * g.cs;
             const float sa = sinf(a), ca = cosf(a);
             float ss = 0.f, sc_ = 0.f, cs_ = 0.f, cc = 0.f;  // (sum, comp) for sin and cos
+            unsigned kmin = 0xFFFFFFFFu, kmax = 0u;           // bounding box (mnk / mxk)
             for (int j = j0 + lane; j < j1; j += kWave) {
                 const float x = Xs[(long long)l * n + j];
+                kmin = min(kmin, ord_key(x));
+                kmax = max(kmax, ord_key(x));
                 const float ts = sinf(x) - sa;
                 const float tc = cosf(x) - ca;
                 float y = ts - sc_;
@@ -465,6 +472,17 @@ __global__ __launch_bounds__(kBlock) void k_summary(
                 } else {
                     two_sum_add(s1, e1, os1, oe1);
                     two_sum_add(s2, e2, os2, oe2);
+                }
+            }
+            if (mnk) {
+#pragma unroll
+                for (int o = kWave / 2; o > 0; o >>= 1) {
+                    kmin = min(kmin, (unsigned)__shfl_xor((int)kmin, o, kWave));
+                    kmax = max(kmax, (unsigned)__shfl_xor((int)kmax, o, kWave));
+                }
+                if (lane == 0) {
+                    atomicMin(&mnk[(long long)l * n + c], kmin);
+                    atomicMax(&mxk[(long long)l * n + c], kmax);
                 }
             }
             if (lane == 0) {
@@ -569,7 +587,8 @@ __global__ __launch_bounds__(kBlock) void k_sub_runs(const int *__restrict__ sub
 // k < *count, with anchors coords * g.cs.
 static void group_summaries(const Geo &g, GridBufs &b, const int *starts, const int *coords,
                             int *nchunk, int *wi_off, const int *count, int *nwork, float *Ts,
-                            float *Tc, DevState *st_loop, hipStream_t s) {
+                            float *Tc, unsigned *mnk, unsigned *mxk, DevState *st_loop,
+                            hipStream_t s) {
     const int n = g.n;
     const int gb = grid_for(n);
     k_wi_count<<<gb, kBlock, 0, s>>>(starts, count, nchunk, st_loop);
@@ -581,7 +600,7 @@ static void group_summaries(const Geo &g, GridBufs &b, const int *starts, const 
 #define SUM_CASE(DD)                                                                          \
     case DD:                                                                                  \
         k_summary<DD><<<wb, kBlock, 0, s>>>(b.Xs, starts, coords, wi_off, nchunk, g, count,   \
-                                            nwork, Ts, Tc, b.partial, st_loop);               \
+                                            nwork, Ts, Tc, b.partial, mnk, mxk, st_loop);     \
         k_summary_combine<DD><<<gb, kBlock, 0, s>>>(starts, coords, wi_off, nchunk, g, count, \
                                                     Ts, Tc, b.partial, st_loop);              \
         break;
@@ -595,7 +614,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     const int n = g.n;
     const int gb = grid_for(n);
     group_summaries(g, b, b.cell_start, b.cell_g, b.nchunk, b.wi_off, &st->C, &st->W, b.Ts,
-                    b.Tc, st_loop, s);
+                    b.Tc, nullptr, nullptr, st_loop, s);
     // runs of equal positions and the update tiles over them
 #define RUN_CASE(DD) \
     case DD: k_run_heads<DD><<<gb, kBlock, 0, s>>>(b.Xs, n, b.head, st_loop); break;
@@ -624,7 +643,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
         Geo gf = g;
         gf.cs = g.cs * 0.5f;
         group_summaries(gf, b, b.sub_start, b.sub_g, b.snchunk, b.swi_off, &st->S, &st->WS,
-                        b.TsS, b.TcS, st_loop, s);
+                        b.TsS, b.TcS, b.smin, b.smax, st_loop, s);
     }
 }
 

@@ -20,6 +20,16 @@ size_t sort_tmp_ints(int n);
 int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int n, int kw,
                int key_bits, int *hist, int *scan_tmp, const DevState *st, hipStream_t s);
 
+// A heavy streamed range of an update tile: tile t, candidate cell c, the
+// range's runs (runs [r0, r0 + total) if smask is 0, else the runs of
+// sub-cells s0 + k for the bits k of smask), and its work units
+// [ubase, ubase + nunits) of kUnitRuns runs.
+struct HeavyDesc {
+    int t, c, r0, s0;
+    unsigned smask;
+    int total, ubase, nunits;
+};
+
 // Grid build, cell summaries (egg_grid.hip).
 struct GridBufs {
     float *X;            // [d][n] current positions (state)
@@ -50,6 +60,7 @@ struct GridBufs {
     int *sub_g;          // [d][n]
     int *sub_run;        // [n+1]
     float *TsS, *TcS;    // [d][n]
+    unsigned *smin, *smax;  // [d][n] bounding box of each sub-cell's points (ord_key)
     int *snchunk, *swi_off; // [n]
     // update tiles: cells are grouped (consecutive cells of one outer cell,
     // equal inner coordinates but the last, last one in the same block of gw
@@ -60,7 +71,14 @@ struct GridBufs {
     float *rres;         // [d][n] new position of each run
     int *sparse_runs;    // [n]   runs of sparse cell groups
     int *sparse_off;     // [n]   their offsets per group (exclusive scan)
-    int *rdbg;           // [4n]  N / half / border / synced of each run (diagnostics)
+    int *rdbg;           // [4n]  N / half / border / synced of each run (diagnostics;
+                         //       also the saved N / half / border of deferred runs)
+    // heavy streamed ranges (k_update_heavy): their sums, per run, in
+    // fixed point (2^-30): sum w sin v [d], sum w (cos v - 1) [d], count;
+    // nullptr for d > kHeavyMaxD (no deferral)
+    long long *hacc;     // [2d+1][n]
+    HeavyDesc *hdesc;    // [kHeavyCap]
+    int *dtiles;         // [n]   tiles whose runs finish in k_update_finish
     float *Ts, *Tc;      // [d][n]
     float *partial;      // [n + n/kSumChunk + 1][4*d]
     int *pairs;          // [2 * pair_cap] extra-check (p, q) sorted positions

@@ -174,6 +174,8 @@ Layout carve(void *base, const Geo &g) {
     b.sub_run = c.take<int>(n + 1);
     b.TsS = c.take<float>(n * d);
     b.TcS = c.take<float>(n * d);
+    b.smin = c.take<unsigned>(n * d);
+    b.smax = c.take<unsigned>(n * d);
     b.snchunk = c.take<int>(n);
     b.swi_off = c.take<int>(n);
     b.gflag = c.take<int>(n);
@@ -185,6 +187,9 @@ Layout carve(void *base, const Geo &g) {
     b.sparse_runs = c.take<int>(n);
     b.sparse_off = c.take<int>(n);
     b.rdbg = c.take<int>(4 * n);
+    b.hacc = g.d <= kHeavyMaxD ? c.take<long long>((2 * d + 1) * n) : nullptr;
+    b.hdesc = c.take<HeavyDesc>(kHeavyCap);
+    b.dtiles = c.take<int>(n);
     b.Ts = c.take<float>(n * d);
     b.Tc = c.take<float>(n * d);
     b.partial = c.take<float>((n + n / kSumChunk + 1) * 4 * d);

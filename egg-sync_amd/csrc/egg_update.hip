@@ -248,38 +248,237 @@ __device__ __forceinline__ unsigned wave_or(unsigned v) {
     return v;
 }
 
-// Classifies sub-cell sc for a lane at x: 1 if every point of it is within
-// eps (its FP32 squared distance <= thr), 0 if none is, 2 otherwise.  FP32
-// squared distances to the sub-cell box, padded by kSubPad against the
-// rounding of cell boundaries, compared with bounds thr * (1 -+ 64 * 2^-24):
-// this distance and the reference's FP32 distance of any point in the box
-// are each within (d + 2) * 2^-24 relative of their exact values, so for
+// Classifies sub-cell sc for a lane at x against threshold thr (thr_in =
+// thr * (1 - 64 * 2^-24), thr_out = thr * (1 + 64 * 2^-24)): 1 if every
+// point of it has FP32 squared distance <= thr, 0 if none has, 2 otherwise.
+// The box is the bounding box of the sub-cell's points (smin / smax); this
+// FP32 distance to the box and the reference's FP32 distance of any point in
+// it are each within (d + 2) * 2^-24 relative of their exact values, so for
 // d <= 20 the margin of 64 * 2^-24 covers both.
-constexpr float kSubPad = 1e-6f;
-
 template <int D>
-__device__ __forceinline__ int sub_class(const int *__restrict__ sub_g, int n, int sc,
-                                         const float *x, const Geo &g, float thr_in,
-                                         float thr_out) {
+__device__ __forceinline__ int sub_class(const unsigned *__restrict__ smin,
+                                         const unsigned *__restrict__ smax, int n, int sc,
+                                         const float *x, float thr_in, float thr_out) {
     static_assert(2 * D + 4 < 64, "classification margin");
     float mn = 0.f, mx = 0.f;
 #pragma unroll
     for (int l = 0; l < D; l++) {
-        const int f = sub_g[(long long)l * n + sc];
-        const int gl = f >> 1;
-        const float top = gl == g.w - 1 ? fmaxf((float)(gl + 1) * g.cs, 1.0f)
-                                        : (float)(gl + 1) * g.cs;
-        const float lo = (f & 1) ? cell_mid(gl, g.cs) : (float)gl * g.cs;
-        const float hi = (f & 1) ? top : cell_mid(gl, g.cs);
-        const float dlo = lo - kSubPad, dhi = hi + kSubPad;
-        const float ta = dlo - x[l], tb = dhi - x[l];
+        const float lo = ord_val(smin[(long long)l * n + sc]);
+        const float hi = ord_val(smax[(long long)l * n + sc]);
+        const float ta = lo - x[l], tb = hi - x[l];
         const float a = ta * ta, b = tb * tb;
         mx += fmaxf(a, b);
-        mn += x[l] < dlo ? a : (x[l] > dhi ? b : 0.f);
+        mn += x[l] < lo ? a : (x[l] > hi ? b : 0.f);
     }
     if (mx <= thr_in) return 1;
     if (mn >= thr_out) return 0;
     return 2;
+}
+
+// Run of index q in a streamed range: runs [r0, r0 + total) if smask is 0,
+// else the runs of the sub-cells s0 + k of the bits k of smask, in order;
+// -1 past the end.
+__device__ __forceinline__ int stream_run(int q, int total, int r0, int s0, unsigned smask,
+                                          const int *__restrict__ sub_run) {
+    if (!smask) return q < total ? r0 + q : -1;
+    int before = 0, rr = -1;
+    for (unsigned mm = smask; mm; mm &= mm - 1) {
+        const int k = __builtin_ctz(mm);
+        const int a = sub_run[s0 + k], b = sub_run[s0 + k + 1];
+        if (q >= before && q < before + (b - a)) rr = a + q - before;
+        before += b - a;
+    }
+    return rr;
+}
+
+// Runs in a streamed range.
+__device__ __forceinline__ int stream_total(int r0, int r1, int s0, unsigned smask,
+                                            const int *__restrict__ sub_run) {
+    if (!smask) return r1 - r0;
+    int total = 0;
+    for (unsigned mm = smask; mm; mm &= mm - 1) {
+        const int k = __builtin_ctz(mm);
+        total += sub_run[s0 + k + 1] - sub_run[s0 + k];
+    }
+    return total;
+}
+
+// One chunk of a streamed range: runs [base, base + 64) of it go through
+// LDS (each lane loads one run and computes sin v and cos v - 1 of its
+// offset v from the anchor a0); lanes with `acc` test each run against eps
+// and accumulate it, weighted by its length (one rounding per run instead of
+// one per point).
+template <int D>
+__device__ __forceinline__ void stream_chunk(int base, int total, int r0, int s0, unsigned smask,
+                                             const int *__restrict__ sub_run,
+                                             const int *__restrict__ run_start,
+                                             const float *__restrict__ Xs, int n,
+                                             const float (&a0)[D], const float (&x)[D],
+                                             float thr, bool acc, bool any_acc, SrcPt<D> *src,
+                                             int *len, float (&ssv)[D], float (&scm)[D],
+                                             int &m_part) {
+    const int lane = __lane_id();
+    const int rr = stream_run(base + lane, total, r0, s0, smask, sub_run);
+    int my_len = 1;
+    if (rr >= 0) {
+        const int j = run_start[rr];
+        my_len = run_start[rr + 1] - j;
+        len[lane] = my_len;
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            const float xq = Xs[(long long)l * n + j];
+            src[lane].x[l] = xq;
+            if (any_acc) sincosm1_small(xq - a0[l], src[lane].sv[l], src[lane].cv[l]);
+        }
+    }
+    const bool multi = __any(my_len > 1);
+    wave_sync();
+    const int m = min(kWave, total - base);
+    if (acc && !multi) {
+        // single-point runs: four sources per step so their LDS reads overlap
+        int k = 0;
+        for (; k + 4 <= m; k += 4) {
+            float sq[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                float qx[D];
+#pragma unroll
+                for (int l = 0; l < D; l++) qx[l] = src[k + jj].x[l];
+                sq[jj] = 0.f;
+#pragma unroll
+                for (int l = 0; l < D; l++) {
+                    const float df = x[l] - qx[l];
+                    sq[jj] += df * df;
+                }
+            }
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                if (sq[jj] <= thr) {
+#pragma unroll
+                    for (int l = 0; l < D; l++) {
+                        ssv[l] += src[k + jj].sv[l];
+                        scm[l] += src[k + jj].cv[l];
+                    }
+                    m_part += 1;
+                }
+            }
+        }
+        for (; k < m; k++) {
+            const SrcPt<D> &qq = src[k];
+            float sq = 0.f;
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                const float df = x[l] - qq.x[l];
+                sq += df * df;
+            }
+            if (sq <= thr) {
+#pragma unroll
+                for (int l = 0; l < D; l++) {
+                    ssv[l] += qq.sv[l];
+                    scm[l] += qq.cv[l];
+                }
+                m_part += 1;
+            }
+        }
+    } else if (acc) {
+        for (int k = 0; k < m; k++) {
+            const SrcPt<D> &qq = src[k];
+            float sq = 0.f;
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                const float df = x[l] - qq.x[l];
+                sq += df * df;
+            }
+            if (sq <= thr) {
+                const int lk = len[k];
+                const float w = (float)lk;  // exact: lk < 2^24
+#pragma unroll
+                for (int l = 0; l < D; l++) {
+                    const float a = w * qq.sv[l];
+                    const float b = w * qq.cv[l];
+                    ssv[l] += a;
+                    scm[l] += b;
+                }
+                m_part += lk;
+            }
+        }
+    }
+    wave_sync();
+}
+
+// Heavy ranges are summed in fixed point (2^-30 units, int64): integer
+// additions commute, so a run's sums do not depend on which wavefront adds
+// which unit, or in what order.  |sum| <= 2n <= 2^31 keeps within 2^61.
+constexpr double kFix = 0x1p30;
+
+__device__ __forceinline__ unsigned long long to_fix(float v) {
+    return (unsigned long long)__double2ll_rn((double)v * kFix);
+}
+
+__device__ __forceinline__ float from_fix(long long v) { return (float)((double)v / kFix); }
+
+template <int D>
+__device__ __forceinline__ void add_fixed(long long *__restrict__ hacc, int n, int r,
+                                          const float (&ps)[D], const float (&pc)[D], int pm) {
+    unsigned long long *h = (unsigned long long *)hacc;
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        atomicAdd(h + (long long)l * n + r, to_fix(ps[l]));
+        atomicAdd(h + (long long)(D + l) * n + r, to_fix(pc[l]));
+    }
+    atomicAdd(h + (long long)(2 * D) * n + r, (unsigned long long)(long long)pm);
+}
+
+// The tile's per-lane setup: position (the anchor for lanes without a run),
+// anchor a0 -- the centre of the tile's cell in the outer dimensions and of
+// its block of kAnchorBlock cells in the inner ones, the same for every
+// group width that divides kAnchorBlock -- and sin / cos of u = x - a0.
+template <int D>
+__device__ __forceinline__ void tile_setup(const TileLane &tl, const float *__restrict__ Xs,
+                                           const int *__restrict__ cell_g, const Geo &g,
+                                           int (&g0)[D], float (&a0)[D], float (&x)[D],
+                                           float (&su)[D], float (&cu)[D]) {
+    const int n = g.n;
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        g0[l] = cell_g[(long long)l * n + tl.gc0];
+        a0[l] = l < g.od ? ((float)g0[l] + 0.5f) * g.cs
+                         : ((float)(g0[l] & ~(kAnchorBlock - 1)) + 0.5f * kAnchorBlock) * g.cs;
+        x[l] = tl.active ? Xs[(long long)l * n + tl.i] : a0[l];
+        sincos_small(x[l] - a0[l], su[l], cu[l]);
+    }
+}
+
+// Final step of a run's update (EGG_SynC.cu:701-709): new position into
+// rres, diagnostics; returns the local-sync test.
+template <int D>
+__device__ __forceinline__ bool finish_run(const float (&x)[D], const float (&su)[D],
+                                           const float (&cu)[D], const float (&sum)[D],
+                                           const float (&ssv)[D], const float (&scm)[D], int N,
+                                           int m_part, int half, bool border, int own, int n,
+                                           int r, float *__restrict__ rres,
+                                           int *__restrict__ rdbg) {
+    N += m_part;
+    float xo[D];
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        const float p1 = ssv[l] * cu[l];
+        const float p2 = ((float)m_part + scm[l]) * su[l];
+        const float sml = p1 - p2;
+        float tt = sum[l] + sml;
+        tt = tt / (float)N;
+        xo[l] = x[l] + tt;
+    }
+    const bool synced = !((!border && N != own) || (border && N != half));
+#pragma unroll
+    for (int l = 0; l < D; l++) rres[(long long)l * n + r] = xo[l];
+    if (rdbg) {
+        rdbg[4 * (long long)r + 0] = N;
+        rdbg[4 * (long long)r + 1] = half;
+        rdbg[4 * (long long)r + 2] = border ? 1 : 0;
+        rdbg[4 * (long long)r + 3] = synced ? 1 : 0;
+    }
+    return synced;
 }
 
 // kernel_itr_grid_STAD5_5_1 (EGG_SynC.cu:551-763).  Per lane: full cells add
@@ -302,9 +501,12 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     const int *__restrict__ cell_run, const int *__restrict__ grp_cell,
     const int *__restrict__ point_cell, const int *__restrict__ tile_off,
     const int *__restrict__ cell_sub, const int *__restrict__ sub_start,
-    const int *__restrict__ sub_g, const int *__restrict__ sub_run,
-    const float *__restrict__ TsS, const float *__restrict__ TcS,
-    float *__restrict__ rres, int *__restrict__ rdbg, Geo g, DevState *st) {
+    const unsigned *__restrict__ smin, const unsigned *__restrict__ smax,
+    const int *__restrict__ sub_run, const float *__restrict__ TsS,
+    const float *__restrict__ TcS, float *__restrict__ rres, int *__restrict__ rdbg,
+    long long *__restrict__ hacc,
+    HeavyDesc *__restrict__ hdesc, int *__restrict__ dtiles, int *__restrict__ dsave, Geo g,
+    DevState *st) {
     if (st->done) return;
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
     __shared__ int s_len[kBlock / kWave][kWave];
@@ -315,6 +517,8 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     // margin of at least 63 * 2^-24
     const float thr_in = thr * (1.0f - 64.0f * 0x1p-24f);
     const float thr_out = thr * (1.0f + 64.0f * 0x1p-24f);
+    const float thr_half_in = thr_half * (1.0f - 64.0f * 0x1p-24f);
+    const float thr_half_out = thr_half * (1.0f + 64.0f * 0x1p-24f);
     const int lane = __lane_id();
     SrcPt<D> *src = s_src[threadIdx.x / kWave];
     int *len = s_len[threadIdx.x / kWave];
@@ -326,163 +530,92 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
         if (sparse_group(cell_run[tl.gc1] - cell_run[tl.gc0], st->runs, st->C))
             continue;  // k_update_sparse
-        const int c0 = tl.gc0, cl = tl.cl, r = tl.r, i = tl.i;
+        const int cl = tl.cl, r = tl.r;
         const bool active = tl.active;
         const int own = cell_start[cl + 1] - cell_start[cl];
         const int ghi = cell_g[(long long)(D - 1) * n + tl.gc1 - 1];
         float x[D], a0[D], sx[D], cx[D], su[D], cu[D], sum[D], ssv[D], scm[D];
         int g0[D];
+        tile_setup<D>(tl, Xs, cell_g, g, g0, a0, x, su, cu);
         bool border = false;
 #pragma unroll
         for (int l = 0; l < D; l++) {
-            g0[l] = cell_g[(long long)l * n + c0];
-            // anchor: the centre of the tile's cell in the outer dimensions
-            // and of its block of kAnchorBlock cells in the inner ones, the
-            // same for every group width that divides kAnchorBlock
-            a0[l] = l < od ? ((float)g0[l] + 0.5f) * g.cs
-                           : ((float)(g0[l] & ~(kAnchorBlock - 1)) + 0.5f * kAnchorBlock) * g.cs;
-            x[l] = active ? Xs[(long long)l * n + i] : a0[l];
             sum[l] = 0.f;
             ssv[l] = 0.f;
             scm[l] = 0.f;
             sx[l] = sinf(x[l]);
             cx[l] = cosf(x[l]);
-            const float u = x[l] - a0[l];
-            sincos_small(u, su[l], cu[l]);
             border |= active && on_border(x[l], g.cs);
         }
         const int g0od = select_dim<D>(g0, od);
         int N = 0, half = 0, m_part = 0;
-        // Streams runs through LDS, 64 per chunk: runs [r0, r1) if smask is
-        // 0, else the runs of the sub-cells s0 + k of the bits k of smask, in
-        // order.  Lanes with `acc` test each run against eps and accumulate
-        // it, weighted by its length (one rounding per run instead of one per
-        // point).
+        bool deferred = false;  // some range went to k_update_heavy
+        int cur_c = 0;          // candidate cell of the range being streamed
+        // Streams a range of runs (stream_run) through LDS, 64 per chunk.  A
+        // range of kHeavyRuns runs or more is queued for k_update_heavy
+        // instead, in units any wavefront can take; the tile's runs then
+        // finish in k_update_finish.
         auto scan_runs = [&](int r0, int r1, int s0, unsigned smask,
                              bool acc) __attribute__((always_inline)) {
             const bool any_acc = __any(acc);
-            int total = r1 - r0;
-            if (smask) {
-                total = 0;
-                for (unsigned mm = smask; mm; mm &= mm - 1) {
-                    const int k = __builtin_ctz(mm);
-                    total += sub_run[s0 + k + 1] - sub_run[s0 + k];
+            const int total = stream_total(r0, r1, s0, smask, sub_run);
+            const bool heavy = hacc && total >= kHeavyRuns;
+            if (heavy) {
+                if (!deferred) {
+                    deferred = true;
+                    if (active) {
+#pragma unroll
+                        for (int k = 0; k < 2 * D + 1; k++) hacc[(long long)k * n + r] = 0;
+                    }
+                    if (lane == 0) dtiles[atomicAdd(&st->ndt, 1)] = t;
                 }
+                // the float sums so far go to hacc now, so that what follows
+                // does not depend on who sums the range (all sums are additive)
+                if (active) add_fixed<D>(hacc, n, r, ssv, scm, m_part);
+#pragma unroll
+                for (int l = 0; l < D; l++) ssv[l] = scm[l] = 0.f;
+                m_part = 0;
+                const int nunits = (total + kUnitRuns - 1) / kUnitRuns;
+                unsigned long long old = 0;
+                if (lane == 0)
+                    old = atomicAdd(&st->heavy_pack, (1ull << 32) | (unsigned long long)nunits);
+                const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)old);
+                const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(old >> 32));
+                if (hi < (unsigned)kHeavyCap) {
+                    if (lane == 0)
+                        hdesc[hi] = HeavyDesc{t, cur_c, r0, s0, smask, total, (int)lo, nunits};
+                    return;
+                }
+                // queue full: this wavefront sums the range itself, unit by
+                // unit in fixed point, as k_update_heavy would
             }
             for (int base = 0; base < total; base += kWave) {
-                const int q = base + lane;
-                int rr = -1;
-                if (!smask) {
-                    if (q < total) rr = r0 + q;
-                } else {
-                    int before = 0;
-                    for (unsigned mm = smask; mm; mm &= mm - 1) {
-                        const int k = __builtin_ctz(mm);
-                        const int a = sub_run[s0 + k], b = sub_run[s0 + k + 1];
-                        if (q >= before && q < before + (b - a)) rr = a + q - before;
-                        before += b - a;
-                    }
+                stream_chunk<D>(base, total, r0, s0, smask, sub_run, run_start, Xs, n, a0, x,
+                                thr, acc, any_acc, src, len, ssv, scm, m_part);
+                if (heavy && (base + kWave == total || (base + kWave) % kUnitRuns == 0 ||
+                              base + kWave > total)) {
+                    if (active) add_fixed<D>(hacc, n, r, ssv, scm, m_part);
+#pragma unroll
+                    for (int l = 0; l < D; l++) ssv[l] = scm[l] = 0.f;
+                    m_part = 0;
                 }
-                int my_len = 1;
-                if (rr >= 0) {
-                    const int j = run_start[rr];
-                    my_len = run_start[rr + 1] - j;
-                    len[lane] = my_len;
-#pragma unroll
-                    for (int l = 0; l < D; l++) {
-                        const float xq = Xs[(long long)l * n + j];
-                        src[lane].x[l] = xq;
-                        if (any_acc) sincosm1_small(xq - a0[l], src[lane].sv[l], src[lane].cv[l]);
-                    }
-                }
-                const bool multi = __any(my_len > 1);
-                wave_sync();
-                const int m = min(kWave, total - base);
-                if (acc && !multi) {
-                    // single-point runs: four sources per step so their
-                    // LDS reads overlap
-                    int k = 0;
-                    for (; k + 4 <= m; k += 4) {
-                        float sq[4];
-#pragma unroll
-                        for (int jj = 0; jj < 4; jj++) {
-                            float qx[D];
-#pragma unroll
-                            for (int l = 0; l < D; l++) qx[l] = src[k + jj].x[l];
-                            sq[jj] = 0.f;
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                const float df = x[l] - qx[l];
-                                sq[jj] += df * df;
-                            }
-                        }
-#pragma unroll
-                        for (int jj = 0; jj < 4; jj++) {
-                            if (sq[jj] <= thr) {
-#pragma unroll
-                                for (int l = 0; l < D; l++) {
-                                    ssv[l] += src[k + jj].sv[l];
-                                    scm[l] += src[k + jj].cv[l];
-                                }
-                                m_part += 1;
-                            }
-                        }
-                    }
-                    for (; k < m; k++) {
-                        const SrcPt<D> &qq = src[k];
-                        float sq = 0.f;
-#pragma unroll
-                        for (int l = 0; l < D; l++) {
-                            const float df = x[l] - qq.x[l];
-                            sq += df * df;
-                        }
-                        if (sq <= thr) {
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                ssv[l] += qq.sv[l];
-                                scm[l] += qq.cv[l];
-                            }
-                            m_part += 1;
-                        }
-                    }
-                } else if (acc) {
-                    for (int k = 0; k < m; k++) {
-                        const SrcPt<D> &qq = src[k];
-                        float sq = 0.f;
-#pragma unroll
-                        for (int l = 0; l < D; l++) {
-                            const float df = x[l] - qq.x[l];
-                            sq += df * df;
-                        }
-                        if (sq <= thr) {
-                            const int lk = len[k];
-                            const float w = (float)lk;  // exact: lk < 2^24
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                const float a = w * qq.sv[l];
-                                const float b = w * qq.cv[l];
-                                ssv[l] += a;
-                                scm[l] += b;
-                            }
-                            m_part += lk;
-                        }
-                    }
-                }
-                wave_sync();
             }
         };
-        // Lanes with `hcnt` count the points of runs [r0, r1) within eps/2.
-        auto half_scan = [&](int r0, int r1, bool hcnt) __attribute__((always_inline)) {
-            for (int base = r0; base < r1; base += kWave) {
-                const int rr = base + lane;
-                if (rr < r1) {
+        // Lanes with `hcnt` count the points of a range of runs (stream_run)
+        // within eps/2.
+        auto half_scan = [&](int r0, int r1, int s0, unsigned smask,
+                             bool hcnt) __attribute__((always_inline)) {
+            const int total = stream_total(r0, r1, s0, smask, sub_run);
+            for (int base = 0; base < total; base += kWave) {
+                const int rr = stream_run(base + lane, total, r0, s0, smask, sub_run);
+                if (rr >= 0) {
                     const int j = run_start[rr];
                     len[lane] = run_start[rr + 1] - j;
 #pragma unroll
                     for (int l = 0; l < D; l++) src[lane].x[l] = Xs[(long long)l * n + j];
                 }
                 wave_sync();
-                const int m = min(kWave, r1 - base);
+                const int m = min(kWave, total - base);
                 if (hcnt) {
                     for (int k = 0; k < m; k++) {
                         float sq = 0.f;
@@ -525,7 +658,32 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                 const bool need_half = full && border && c != cl;
                 const bool any_part = __any(part), any_half = __any(need_half);
                 if (!any_part && !any_half) continue;
-                if (any_half) half_scan(cell_run[c], cell_run[c + 1], need_half);
+                if (any_half) {
+                    // eps/2-neighbours in another full cell: sub-cells wholly
+                    // within eps/2 of a lane count whole, those straddling
+                    // eps/2 for some lane are streamed for the wavefront
+                    const int hs0 = cell_sub[c], hns = cell_sub[c + 1] - hs0;
+                    if (g.sub_bits && hns <= 32) {
+                        unsigned hin = 0, hst = 0;
+                        if (need_half) {
+                            for (int k = 0; k < hns; k++) {
+                                const int cls = sub_class<D>(smin, smax, n, hs0 + k, x,
+                                                             thr_half_in, thr_half_out);
+                                if (cls == 1) hin |= 1u << k;
+                                else if (cls == 2) hst |= 1u << k;
+                            }
+                        }
+                        const unsigned hu = wave_or(hst);
+                        const unsigned ht = hin & ~hu;
+                        for (unsigned mm = wave_or(ht); mm; mm &= mm - 1) {
+                            const int k = __builtin_ctz(mm);
+                            if ((ht >> k) & 1u) half += sub_start[hs0 + k + 1] - sub_start[hs0 + k];
+                        }
+                        if (hu) half_scan(0, 0, hs0, hu, need_half);
+                    } else {
+                        half_scan(cell_run[c], cell_run[c + 1], 0, 0u, need_half);
+                    }
+                }
                 if (!any_part) continue;
                 const int s0 = cell_sub[c], ns = cell_sub[c + 1] - s0;
                 unsigned umask = 0;
@@ -539,7 +697,8 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                     unsigned imask = 0, smask = 0;
                     if (part) {
                         for (int k = 0; k < ns; k++) {
-                            const int cls = sub_class<D>(sub_g, n, s0 + k, x, g, thr_in, thr_out);
+                            const int cls = sub_class<D>(smin, smax, n, s0 + k, x, thr_in,
+                                                         thr_out);
                             if (cls == 1) imask |= 1u << k;
                             else if (cls == 2) smask |= 1u << k;
                         }
@@ -562,34 +721,124 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                     }
                     if (!umask) continue;
                 }
+                cur_c = c;
                 scan_runs(cell_run[c], cell_run[c + 1], s0, umask, part);
             }
         });
-        N += m_part;
-        float xo[D];
+        if (deferred) {
+            // the sums so far wait for the heavy ranges: full-cell totals in
+            // rres, N / half / border in dsave, streamed sums into hacc
+            if (active) {
 #pragma unroll
-        for (int l = 0; l < D; l++) {
-            const float p1 = ssv[l] * cu[l];
-            const float p2 = ((float)m_part + scm[l]) * su[l];
-            const float sml = p1 - p2;
-            float tt = sum[l] + sml;
-            tt = tt / (float)N;
-            xo[l] = x[l] + tt;
-        }
-        const bool synced = !((!border && N != own) || (border && N != half));
-        if (active) {
-#pragma unroll
-            for (int l = 0; l < D; l++) rres[(long long)l * n + r] = xo[l];
-            if (!synced) unsynced = true;
-            if (rdbg) {
-                rdbg[4 * (long long)r + 0] = N;
-                rdbg[4 * (long long)r + 1] = half;
-                rdbg[4 * (long long)r + 2] = border ? 1 : 0;
-                rdbg[4 * (long long)r + 3] = synced ? 1 : 0;
+                for (int l = 0; l < D; l++) rres[(long long)l * n + r] = sum[l];
+                dsave[4 * (long long)r + 0] = N;
+                dsave[4 * (long long)r + 1] = half;
+                dsave[4 * (long long)r + 2] = border ? 1 : 0;
+                add_fixed<D>(hacc, n, r, ssv, scm, m_part);
             }
+            continue;
+        }
+        if (active) {
+            if (!finish_run<D>(x, su, cu, sum, ssv, scm, N, m_part, half, border, own, n, r, rres,
+                               rdbg))
+                unsynced = true;
         }
     }
     if (__any(unsynced) && lane == 0) atomicAnd(&st->flag, 0);
+}
+
+// Work units of the heavy ranges (kUnitRuns runs each), from a queue over
+// all of them: every wavefront rebuilds the tile's lanes, tests the unit's
+// runs as the tile kernel would and adds its sums in fixed point.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_update_heavy(
+    const float *__restrict__ Xs, const int *__restrict__ cell_g,
+    const int *__restrict__ run_start, const int *__restrict__ cell_run,
+    const int *__restrict__ grp_cell, const int *__restrict__ point_cell,
+    const int *__restrict__ tile_off, const int *__restrict__ sub_run,
+    const HeavyDesc *__restrict__ hdesc, long long *__restrict__ hacc, Geo g, DevState *st) {
+    if (st->done) return;
+    const int nd = (int)min((unsigned long long)kHeavyCap, st->heavy_pack >> 32);
+    if (nd == 0) return;
+    __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
+    __shared__ int s_len[kBlock / kWave][kWave];
+    SrcPt<D> *src = s_src[threadIdx.x / kWave];
+    int *len = s_len[threadIdx.x / kWave];
+    const int n = g.n, G = st->G;
+    const float thr = g.thr_eps;
+    // descriptors k < nd hold consecutive unit ranges in the order of k
+    const int end = hdesc[nd - 1].ubase + hdesc[nd - 1].nunits;
+    for (;;) {
+        const int u = next_tile(&st->unit_next);
+        if (u >= end) break;
+        int lo = 0, hi = nd - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (hdesc[mid].ubase <= u) lo = mid;
+            else hi = mid - 1;
+        }
+        const HeavyDesc hd = hdesc[lo];
+        const TileLane tl = tile_lane(hd.t, tile_off, G, grp_cell, cell_run, run_start,
+                                      point_cell);
+        float x[D], a0[D], su[D], cu[D];
+        int g0[D], gc[D];
+        tile_setup<D>(tl, Xs, cell_g, g, g0, a0, x, su, cu);
+#pragma unroll
+        for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + hd.c];
+        float sm, lg;
+        box_dist<D>(gc, x, g.cs, sm, lg);
+        const bool acc = tl.active && !(lg <= thr) && sm <= thr;
+        const bool any_acc = __any(acc);
+        float ps[D], pc[D];
+        int pm = 0;
+#pragma unroll
+        for (int l = 0; l < D; l++) ps[l] = pc[l] = 0.f;
+        const int k = u - hd.ubase;
+        const int b1 = min(hd.total, (k + 1) * kUnitRuns);
+        for (int base = k * kUnitRuns; base < b1; base += kWave)
+            stream_chunk<D>(base, b1, hd.r0, hd.s0, hd.smask, sub_run, run_start, Xs, n, a0, x,
+                            thr, acc, any_acc, src, len, ps, pc, pm);
+        if (acc) add_fixed<D>(hacc, n, tl.r, ps, pc, pm);
+    }
+}
+
+// The runs of tiles with heavy ranges: their sums from hacc, then the
+// tile kernel's final step.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_update_finish(
+    const float *__restrict__ Xs, const int *__restrict__ cell_start,
+    const int *__restrict__ cell_g, const int *__restrict__ run_start,
+    const int *__restrict__ cell_run, const int *__restrict__ grp_cell,
+    const int *__restrict__ point_cell, const int *__restrict__ tile_off,
+    const int *__restrict__ dtiles, const int *__restrict__ dsave,
+    const long long *__restrict__ hacc, float *__restrict__ rres, int *__restrict__ rdbg, Geo g,
+    DevState *st) {
+    if (st->done) return;
+    const int ndt = st->ndt, n = g.n, G = st->G;
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    bool unsynced = false;
+    for (int w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; w < ndt; w += waves) {
+        const TileLane tl = tile_lane(dtiles[w], tile_off, G, grp_cell, cell_run, run_start,
+                                      point_cell);
+        if (!tl.active) continue;
+        const int r = tl.r;
+        float x[D], a0[D], su[D], cu[D], sum[D], ssv[D], scm[D];
+        int g0[D];
+        tile_setup<D>(tl, Xs, cell_g, g, g0, a0, x, su, cu);
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            sum[l] = rres[(long long)l * n + r];
+            ssv[l] = from_fix(hacc[(long long)l * n + r]);
+            scm[l] = from_fix(hacc[(long long)(D + l) * n + r]);
+        }
+        const int m_part = (int)hacc[(long long)(2 * D) * n + r];
+        const int own = cell_start[tl.cl + 1] - cell_start[tl.cl];
+        if (!finish_run<D>(x, su, cu, sum, ssv, scm, dsave[4 * (long long)r],
+                           m_part, dsave[4 * (long long)r + 1], dsave[4 * (long long)r + 2] != 0,
+                           own, n, r, rres, rdbg))
+            unsynced = true;
+    }
+    if (__any(unsynced) && __lane_id() == 0) atomicAnd(&st->flag, 0);
 }
 
 // The update for the runs of sparse cell groups (sparse_group; the tile
@@ -1073,6 +1322,9 @@ __global__ void k_set_flag(DevState *st) {
     st->pair_count = 0;
     st->tile_next = 0;
     st->xtile_next = 0;
+    st->heavy_pack = 0;
+    st->unit_next = 0;
+    st->ndt = 0;
 }
 
 // End of an iteration (EGG_SynC.cu:1322,1425 / SynC.cpp:277).
@@ -1122,8 +1374,19 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
         k_update_tile<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc,     \
                                             b.ob, b.oe, b.run_start, b.cell_run,              \
                                             b.grp_cell, b.point_cell, b.tile_off,             \
-                                            b.cell_sub, b.sub_start, b.sub_g, b.sub_run,      \
-                                            b.TsS, b.TcS, b.rres, rdbg, g, st);               \
+                                            b.cell_sub, b.sub_start, b.smin, b.smax,          \
+                                            b.sub_run, b.TsS, b.TcS, b.rres, rdbg, b.hacc,    \
+                                            b.hdesc,                                          \
+                                            b.dtiles, b.rdbg, g, st);                         \
+        if (b.hacc) {                                                                         \
+            k_update_heavy<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_g, b.run_start, b.cell_run,     \
+                                                 b.grp_cell, b.point_cell, b.tile_off,        \
+                                                 b.sub_run, b.hdesc, b.hacc, g, st);          \
+            k_update_finish<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.run_start,  \
+                                                  b.cell_run, b.grp_cell, b.point_cell,       \
+                                                  b.tile_off, b.dtiles, b.rdbg, b.hacc,       \
+                                                  b.rres, rdbg, g, st);                       \
+        }                                                                                     \
         k_update_sparse<DD><<<grid_for(g.n), kBlock, 0, s>>>(                                \
             b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.ob, b.oe, b.run_start, b.cell_run,      \
             b.point_cell, b.sparse_runs, b.rres, rdbg, g, st);                                 \
