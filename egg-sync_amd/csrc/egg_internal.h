@@ -28,6 +28,7 @@ struct HeavyDesc {
     int t, c, r0, s0;
     unsigned smask;
     int total, ubase, nunits;
+    int half;  // 1: an eps/2 count over a full cell (border lanes), 0: the update's sums
 };
 
 // Grid build, cell summaries (egg_grid.hip).
@@ -74,9 +75,9 @@ struct GridBufs {
     int *rdbg;           // [4n]  N / half / border / synced of each run (diagnostics;
                          //       also the saved N / half / border of deferred runs)
     // heavy streamed ranges (k_update_heavy): their sums, per run, in
-    // fixed point (2^-30): sum w sin v [d], sum w (cos v - 1) [d], count;
-    // nullptr for d > kHeavyMaxD (no deferral)
-    long long *hacc;     // [2d+1][n]
+    // fixed point (2^-30): sum w sin v [d], sum w (cos v - 1) [d], count,
+    // and the eps/2 count; nullptr for d > kHeavyMaxD (no deferral)
+    long long *hacc;     // [2d+2][n]
     HeavyDesc *hdesc;    // [kHeavyCap]
     int *dtiles;         // [n]   tiles whose runs finish in k_update_finish
     float *Ts, *Tc;      // [d][n]

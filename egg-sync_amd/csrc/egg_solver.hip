@@ -187,7 +187,7 @@ Layout carve(void *base, const Geo &g) {
     b.sparse_runs = c.take<int>(n);
     b.sparse_off = c.take<int>(n);
     b.rdbg = c.take<int>(4 * n);
-    b.hacc = g.d <= kHeavyMaxD ? c.take<long long>((2 * d + 1) * n) : nullptr;
+    b.hacc = g.d <= kHeavyMaxD ? c.take<long long>((2 * d + 2) * n) : nullptr;
     b.hdesc = c.take<HeavyDesc>(kHeavyCap);
     b.dtiles = c.take<int>(n);
     b.Ts = c.take<float>(n * d);

@@ -551,6 +551,30 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         int N = 0, half = 0, m_part = 0;
         bool deferred = false;  // some range went to k_update_heavy
         int cur_c = 0;          // candidate cell of the range being streamed
+        auto defer_tile = [&]() __attribute__((always_inline)) {
+            if (!deferred) {
+                deferred = true;
+                if (active) {
+#pragma unroll
+                    for (int k = 0; k < 2 * D + 2; k++) hacc[(long long)k * n + r] = 0;
+                }
+                if (lane == 0) dtiles[atomicAdd(&st->ndt, 1)] = t;
+            }
+        };
+        // queues a heavy range; false if the queue is full
+        auto queue_range = [&](int r0, int s0, unsigned smask, int total,
+                               int half_kind) __attribute__((always_inline)) {
+            const int nunits = (total + kUnitRuns - 1) / kUnitRuns;
+            unsigned long long old = 0;
+            if (lane == 0)
+                old = atomicAdd(&st->heavy_pack, (1ull << 32) | (unsigned long long)nunits);
+            const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)old);
+            const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(old >> 32));
+            if (hi >= (unsigned)kHeavyCap) return false;
+            if (lane == 0)
+                hdesc[hi] = HeavyDesc{t, cur_c, r0, s0, smask, total, (int)lo, nunits, half_kind};
+            return true;
+        };
         // Streams a range of runs (stream_run) through LDS, 64 per chunk.  A
         // range of kHeavyRuns runs or more is queued for k_update_heavy
         // instead, in units any wavefront can take; the tile's runs then
@@ -561,31 +585,14 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
             const int total = stream_total(r0, r1, s0, smask, sub_run);
             const bool heavy = hacc && total >= kHeavyRuns;
             if (heavy) {
-                if (!deferred) {
-                    deferred = true;
-                    if (active) {
-#pragma unroll
-                        for (int k = 0; k < 2 * D + 1; k++) hacc[(long long)k * n + r] = 0;
-                    }
-                    if (lane == 0) dtiles[atomicAdd(&st->ndt, 1)] = t;
-                }
+                defer_tile();
                 // the float sums so far go to hacc now, so that what follows
                 // does not depend on who sums the range (all sums are additive)
                 if (active) add_fixed<D>(hacc, n, r, ssv, scm, m_part);
 #pragma unroll
                 for (int l = 0; l < D; l++) ssv[l] = scm[l] = 0.f;
                 m_part = 0;
-                const int nunits = (total + kUnitRuns - 1) / kUnitRuns;
-                unsigned long long old = 0;
-                if (lane == 0)
-                    old = atomicAdd(&st->heavy_pack, (1ull << 32) | (unsigned long long)nunits);
-                const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)old);
-                const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(old >> 32));
-                if (hi < (unsigned)kHeavyCap) {
-                    if (lane == 0)
-                        hdesc[hi] = HeavyDesc{t, cur_c, r0, s0, smask, total, (int)lo, nunits};
-                    return;
-                }
+                if (queue_range(r0, s0, smask, total, 0)) return;
                 // queue full: this wavefront sums the range itself, unit by
                 // unit in fixed point, as k_update_heavy would
             }
@@ -606,6 +613,11 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         auto half_scan = [&](int r0, int r1, int s0, unsigned smask,
                              bool hcnt) __attribute__((always_inline)) {
             const int total = stream_total(r0, r1, s0, smask, sub_run);
+            if (hacc && total >= kHeavyRuns) {
+                // a count: exact in any order, so k_update_heavy may take it
+                defer_tile();
+                if (queue_range(r0, s0, smask, total, 1)) return;
+            }
             for (int base = 0; base < total; base += kWave) {
                 const int rr = stream_run(base + lane, total, r0, s0, smask, sub_run);
                 if (rr >= 0) {
@@ -659,6 +671,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                 const bool any_part = __any(part), any_half = __any(need_half);
                 if (!any_part && !any_half) continue;
                 if (any_half) {
+                    cur_c = c;
                     // eps/2-neighbours in another full cell: sub-cells wholly
                     // within eps/2 of a lane count whole, those straddling
                     // eps/2 for some lane are streamed for the wavefront
@@ -787,14 +800,52 @@ __global__ __launch_bounds__(kBlock) void k_update_heavy(
         for (int l = 0; l < D; l++) gc[l] = cell_g[(long long)l * n + hd.c];
         float sm, lg;
         box_dist<D>(gc, x, g.cs, sm, lg);
+        const int k = u - hd.ubase;
+        const int b1 = min(hd.total, (k + 1) * kUnitRuns);
+        if (hd.half) {
+            // border lanes of the tile, the range's cell full for them and
+            // not their own: points within eps/2
+            bool border = false;
+#pragma unroll
+            for (int l = 0; l < D; l++) border |= on_border(x[l], g.cs);
+            const bool hcnt = tl.active && lg <= thr && border && hd.c != tl.cl;
+            int cnt = 0;
+            for (int base = k * kUnitRuns; base < b1; base += kWave) {
+                const int rr = stream_run(base + __lane_id(), b1, hd.r0, hd.s0, hd.smask,
+                                          sub_run);
+                if (rr >= 0) {
+                    const int j = run_start[rr];
+                    len[__lane_id()] = run_start[rr + 1] - j;
+#pragma unroll
+                    for (int l = 0; l < D; l++)
+                        src[__lane_id()].x[l] = Xs[(long long)l * n + j];
+                }
+                wave_sync();
+                const int m = min(kWave, b1 - base);
+                if (hcnt) {
+                    for (int q = 0; q < m; q++) {
+                        float sq = 0.f;
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            const float df = x[l] - src[q].x[l];
+                            sq += df * df;
+                        }
+                        if (sq <= g.thr_half) cnt += len[q];
+                    }
+                }
+                wave_sync();
+            }
+            if (hcnt)
+                atomicAdd((unsigned long long *)hacc + (long long)(2 * D + 1) * n + tl.r,
+                          (unsigned long long)cnt);
+            continue;
+        }
         const bool acc = tl.active && !(lg <= thr) && sm <= thr;
         const bool any_acc = __any(acc);
         float ps[D], pc[D];
         int pm = 0;
 #pragma unroll
         for (int l = 0; l < D; l++) ps[l] = pc[l] = 0.f;
-        const int k = u - hd.ubase;
-        const int b1 = min(hd.total, (k + 1) * kUnitRuns);
         for (int base = k * kUnitRuns; base < b1; base += kWave)
             stream_chunk<D>(base, b1, hd.r0, hd.s0, hd.smask, sub_run, run_start, Xs, n, a0, x,
                             thr, acc, any_acc, src, len, ps, pc, pm);
@@ -833,9 +884,9 @@ __global__ __launch_bounds__(kBlock) void k_update_finish(
         }
         const int m_part = (int)hacc[(long long)(2 * D) * n + r];
         const int own = cell_start[tl.cl + 1] - cell_start[tl.cl];
-        if (!finish_run<D>(x, su, cu, sum, ssv, scm, dsave[4 * (long long)r],
-                           m_part, dsave[4 * (long long)r + 1], dsave[4 * (long long)r + 2] != 0,
-                           own, n, r, rres, rdbg))
+        const int half = dsave[4 * (long long)r + 1] + (int)hacc[(long long)(2 * D + 1) * n + r];
+        if (!finish_run<D>(x, su, cu, sum, ssv, scm, dsave[4 * (long long)r], m_part, half,
+                           dsave[4 * (long long)r + 2] != 0, own, n, r, rres, rdbg))
             unsynced = true;
     }
     if (__any(unsynced) && __lane_id() == 0) atomicAnd(&st->flag, 0);
