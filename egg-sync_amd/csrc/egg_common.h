@@ -33,7 +33,7 @@ constexpr int kDefaultGroup = 1;  // default cells per tile group along the last
 constexpr int kAnchorBlock = 8;   // group widths are powers of two dividing this
 constexpr int kSparseRuns = 8;    // cell groups with fewer runs are updated one lane per run,
 constexpr int kSparseFactor = 4;  // in iterations with fewer than kSparseFactor runs per cell
-constexpr int kHeavyRuns = 8192;  // a streamed range of at least this many runs is heavy:
+constexpr int kHeavyRuns = 4096;  // a streamed range of at least this many runs is heavy:
 constexpr int kUnitRuns = 1024;   // split into units of this many runs, any wavefront's work,
 constexpr int kHeavyCap = 65536;  // with up to this many ranges per pass in the queue,
 constexpr int kHeavyMaxD = 8;     // for d up to this (per-run fixed-point sums: (2d+2)*8 B)
