@@ -33,12 +33,14 @@ sys.path.insert(0, os.path.join(ROOT, "egg-sync_amd"))
 
 METRIC = "time-to-clustering (s), point-updates/s; plain vs learned form, same labels"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_update_tile<3> launch at n=1M (mean over the solve's 10
-# launches): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each divided by what
-# the counter reads for a 256 MiB copy (2x and 4x the bytes moved); see
-# profiles/k_update_tile_config1_n1M_counters.md.  Measured for this workload
+# HBM bytes per update pass at n=1M (mean over the solve's 10 passes; the
+# pass is k_update_tile<3> and the kernels launched with it: k_update_heavy,
+# k_update_finish, k_update_sparse, k_scatter_runs): rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE, each divided by what the counter reads for a
+# 256 MiB copy (2x and 4x the bytes moved); see
+# profiles/update_pass_config1_n1M_counters.md.  Measured for this workload
 # only, so reported only when n is 1,000,000.
-UPDATE_TRAFFIC_BYTES_N1M = 47.8e6
+UPDATE_TRAFFIC_BYTES_N1M = 86.6e6
 
 
 def cpu_baseline(X, eps, target_seconds=12.0):
@@ -278,7 +280,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": UPDATE_TRAFFIC_BYTES_N1M if n == 1_000_000 else None,
-                     "kernel": "k_update_tile<3>", "launches": launches,
+                     "kernel": "update pass: k_update_tile<3> + k_update_heavy<3> + "
+                               "k_update_finish<3> + k_update_sparse<3> + k_scatter_runs<3>",
+                     "launches": launches,
                      "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }

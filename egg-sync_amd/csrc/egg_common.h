@@ -36,7 +36,7 @@ constexpr int kSparseFactor = 4;  // in iterations with fewer than kSparseFactor
 constexpr int kHeavyRuns = 4096;  // a streamed range of at least this many runs is heavy:
 constexpr int kUnitRuns = 1024;   // split into units of this many runs, any wavefront's work,
 constexpr int kHeavyCap = 65536;  // with up to this many ranges per pass in the queue,
-constexpr int kHeavyMaxD = 8;     // for d up to this (per-run fixed-point sums: (2d+2)*8 B)
+constexpr int kHeavyMaxD = 5;     // for d up to this (per-run fixed-point sums: (2d+2)*8 B)
 
 // Expands X(D) for every dimension count the kernels are instantiated for;
 // host code dispatches on the runtime d with `switch (d) { EGG_EACH_D(X) }`.

@@ -508,6 +508,9 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
     HeavyDesc *__restrict__ hdesc, int *__restrict__ dtiles, int *__restrict__ dsave, Geo g,
     DevState *st) {
     if (st->done) return;
+    // heavy ranges go to k_update_heavy only up to kHeavyMaxD dimensions
+    // (the deferral code costs registers)
+    constexpr bool kDefer = D <= kHeavyMaxD;
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
     __shared__ int s_len[kBlock / kWave][kWave];
     const int n = g.n, R = g.R, od = g.od;
@@ -583,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                              bool acc) __attribute__((always_inline)) {
             const bool any_acc = __any(acc);
             const int total = stream_total(r0, r1, s0, smask, sub_run);
-            const bool heavy = hacc && total >= kHeavyRuns;
+            const bool heavy = kDefer && hacc && total >= kHeavyRuns;
             if (heavy) {
                 defer_tile();
                 // the float sums so far go to hacc now, so that what follows
@@ -613,7 +616,7 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
         auto half_scan = [&](int r0, int r1, int s0, unsigned smask,
                              bool hcnt) __attribute__((always_inline)) {
             const int total = stream_total(r0, r1, s0, smask, sub_run);
-            if (hacc && total >= kHeavyRuns) {
+            if (kDefer && hacc && total >= kHeavyRuns) {
                 // a count: exact in any order, so k_update_heavy may take it
                 defer_tile();
                 if (queue_range(r0, s0, smask, total, 1)) return;
