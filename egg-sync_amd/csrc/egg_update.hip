@@ -248,7 +248,7 @@ __device__ __forceinline__ unsigned wave_or(unsigned v) {
     return v;
 }
 
-// Classifies sub-cell sc for a lane at x against threshold thr (thr_in =
+// Classifies a sub-cell for a lane at x against threshold thr (thr_in =
 // thr * (1 - 64 * 2^-24), thr_out = thr * (1 + 64 * 2^-24)): 1 if every
 // point of it has FP32 squared distance <= thr, 0 if none has, 2 otherwise.
 // The box is the bounding box of the sub-cell's points (smin / smax); this
@@ -256,15 +256,13 @@ __device__ __forceinline__ unsigned wave_or(unsigned v) {
 // it are each within (d + 2) * 2^-24 relative of their exact values, so for
 // d <= 20 the margin of 64 * 2^-24 covers both.
 template <int D>
-__device__ __forceinline__ int sub_class(const unsigned *__restrict__ smin,
-                                         const unsigned *__restrict__ smax, int n, int sc,
-                                         const float *x, float thr_in, float thr_out) {
+__device__ __forceinline__ int sub_class_box(const float *blo, const float *bhi, const float *x,
+                                             float thr_in, float thr_out) {
     static_assert(2 * D + 4 < 64, "classification margin");
     float mn = 0.f, mx = 0.f;
 #pragma unroll
     for (int l = 0; l < D; l++) {
-        const float lo = ord_val(smin[(long long)l * n + sc]);
-        const float hi = ord_val(smax[(long long)l * n + sc]);
+        const float lo = blo[l], hi = bhi[l];
         const float ta = lo - x[l], tb = hi - x[l];
         const float a = ta * ta, b = tb * tb;
         mx += fmaxf(a, b);
@@ -273,6 +271,24 @@ __device__ __forceinline__ int sub_class(const unsigned *__restrict__ smin,
     if (mx <= thr_in) return 1;
     if (mn >= thr_out) return 0;
     return 2;
+}
+
+// The boxes of sub-cells s0 .. s0 + ns - 1 (ns <= 64) into LDS, box k in
+// src[k] (lower corner in x, upper in sv), so that every lane reads them as
+// broadcasts instead of loading them one after another.
+template <int D>
+__device__ __forceinline__ void stage_boxes(const unsigned *__restrict__ smin,
+                                            const unsigned *__restrict__ smax, int n, int s0,
+                                            int ns, SrcPt<D> *src) {
+    const int lane = __lane_id();
+    if (lane < ns) {
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            src[lane].x[l] = ord_val(smin[(long long)l * n + s0 + lane]);
+            src[lane].sv[l] = ord_val(smax[(long long)l * n + s0 + lane]);
+        }
+    }
+    wave_sync();
 }
 
 // Run of index q in a streamed range: runs [r0, r0 + total) if smask is 0,
@@ -681,14 +697,16 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                     const int hs0 = cell_sub[c], hns = cell_sub[c + 1] - hs0;
                     if (g.sub_bits && hns <= 32) {
                         unsigned hin = 0, hst = 0;
+                        stage_boxes<D>(smin, smax, n, hs0, hns, src);
                         if (need_half) {
                             for (int k = 0; k < hns; k++) {
-                                const int cls = sub_class<D>(smin, smax, n, hs0 + k, x,
-                                                             thr_half_in, thr_half_out);
+                                const int cls = sub_class_box<D>(src[k].x, src[k].sv, x,
+                                                                 thr_half_in, thr_half_out);
                                 if (cls == 1) hin |= 1u << k;
                                 else if (cls == 2) hst |= 1u << k;
                             }
                         }
+                        wave_sync();
                         const unsigned hu = wave_or(hst);
                         const unsigned ht = hin & ~hu;
                         for (unsigned mm = wave_or(ht); mm; mm &= mm - 1) {
@@ -711,14 +729,16 @@ __global__ __launch_bounds__(kBlock) void k_update_tile(
                     // a partial cell; other sub-cells wholly within eps add
                     // their totals
                     unsigned imask = 0, smask = 0;
+                    stage_boxes<D>(smin, smax, n, s0, ns, src);
                     if (part) {
                         for (int k = 0; k < ns; k++) {
-                            const int cls = sub_class<D>(smin, smax, n, s0 + k, x, thr_in,
-                                                         thr_out);
+                            const int cls =
+                                sub_class_box<D>(src[k].x, src[k].sv, x, thr_in, thr_out);
                             if (cls == 1) imask |= 1u << k;
                             else if (cls == 2) smask |= 1u << k;
                         }
                     }
+                    wave_sync();
                     umask = wave_or(smask);
                     const unsigned tmask = imask & ~umask;
                     for (unsigned mm = wave_or(tmask); mm; mm &= mm - 1) {
