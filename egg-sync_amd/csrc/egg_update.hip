@@ -509,8 +509,12 @@ __device__ __forceinline__ bool finish_run(const float (&x)[D], const float (&su
 // so the result keeps the accuracy of a direct sum of sinf(x_q - x).  Border
 // lanes count their eps/2-neighbours in other full cells from the same
 // chunks (EGG_SynC.cu:666-681).
+// Up to 3 dimensions the compiler is held to 6 wavefronts per SIMD (80
+// VGPRs, a few spills: the tile's constants); more resident wavefronts hide
+// the LDS and L2 latency of the streaming loop (config 1: -8%).
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_update_tile(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D <= 3 ? 6 : 1)))
+void k_update_tile(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
     const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ run_start,
