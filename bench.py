@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # 256 MiB copy (2x and 4x the bytes moved); see
 # profiles/update_pass_config1_n1M_counters.md.  Measured for this workload
 # only, so reported only when n is 1,000,000.
-UPDATE_TRAFFIC_BYTES_N1M = 86.6e6
+UPDATE_TRAFFIC_BYTES_N1M = 207.5e6
 
 
 def cpu_baseline(X, eps, target_seconds=12.0):
