@@ -206,8 +206,9 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the solve is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+        print(f"bench.py: the solve runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a multi-GPU solve would split the outer grid into slabs along dimension 0 "
+              "with a halo of R outer cells; it is not built)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
