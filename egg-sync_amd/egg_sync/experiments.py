@@ -10,6 +10,7 @@ be repeated, and loaders for the real data sets of python/data.py:46-97 that
 read the same files with the same columns and delimiters.
 
     python -m egg_sync.experiments --vary n --values 2000 4000 8000 --rounds 3
+    python -m egg_sync.experiments --vary real --data-dir <reference>/data
 
 prints one JSON record per (value, round): running time (wall, around the
 call, as run_experiment.py:322-325), per-iteration and per-stage times and
@@ -73,10 +74,36 @@ DATASETS = {
 }
 
 
+# The real data sets of the reference's run_real (run_experiment.py:53-63,547-551).
+REAL = ["banknote", "Yeast", "Wilt", "CCPP", "eb", "Skin_NonSkin", "3D_spatial_network",
+        "EEG_Eye_State", "Letter"]
+
+
 def load_dataset(name: str, data_dir: str):
     """A real data set from `data_dir`, min-max normalised, float32."""
     fname, kw = DATASETS[name]
     return min_max_normalize(np.loadtxt(os.path.join(data_dir, fname), **kw))
+
+
+def run_real(data_dir: str, rounds: int = PARAMS["rounds"], eps: float = PARAMS["eps"],
+             algorithm=None, names=None):
+    """run_experiment.py:547-551: every real data set (those whose file is in
+    `data_dir`; 3D_spatial_network is not shipped with the reference) at the
+    default eps, `rounds` timed calls each."""
+    if algorithm is None:
+        from .algorithms import EGG_SynC as algorithm
+    out = []
+    for name in names or REAL:
+        if not os.path.exists(os.path.join(data_dir, DATASETS[name][0])):
+            continue
+        X = load_dataset(name, data_dir)
+        for r in range(rounds):
+            secs, itr, stages, space = run_one(algorithm, X, eps)
+            out.append({"dataset": name, "round": r, "n": int(X.shape[0]), "d": int(X.shape[1]),
+                        "eps": float(eps), "running_time": secs, "iterations": int(len(itr)),
+                        "stage_times": dict(zip(STAGES, [float(t) for t in stages])),
+                        "space": int(space)})
+    return out
 
 
 def run_one(algorithm, X, eps: float):
@@ -115,7 +142,8 @@ def run(vary: str = "n", values=None, rounds: int = PARAMS["rounds"], algorithm=
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--vary", default="n", choices=sorted(SWEEPS))
+    ap.add_argument("--vary", default="n", choices=sorted(SWEEPS) + ["real"])
+    ap.add_argument("--data-dir", default="data", help="directory of the real data files")
     ap.add_argument("--values", type=float, nargs="*")
     ap.add_argument("--rounds", type=int, default=PARAMS["rounds"])
     for k in ("n", "d", "cl"):
@@ -123,6 +151,10 @@ def main(argv=None):
     for k in ("std", "eps"):
         ap.add_argument(f"--{k}", type=float, default=PARAMS[k])
     a = ap.parse_args(argv)
+    if a.vary == "real":
+        for rec in run_real(a.data_dir, a.rounds, a.eps):
+            print(json.dumps(rec), flush=True)
+        return
     vals = None
     if a.values:
         vals = [int(v) if a.vary in ("n", "d", "cl") else v for v in a.values]

@@ -203,3 +203,16 @@ def test_dataset_loaders_read_the_reference_files():
             continue
         X = E.load_dataset(name, data_dir)
         assert X.ndim == 2 and X.shape[0] > 100 and 0.0 <= X.min() and X.max() <= 1.0, name
+
+
+def test_real_fixture_inputs_are_the_loaders_output():
+    """tests/golden/real_oracle.npz holds the loaders' output for the reference's files."""
+    from egg_sync import experiments as E
+    from conftest import golden
+    data_dir = "/root/reference/data"
+    if not os.path.isdir(data_dir):
+        pytest.skip("reference data files not present")
+    g = golden("real_oracle")
+    for name in ("banknote", "Yeast", "Wilt", "CCPP", "pendigits"):
+        assert np.array_equal(E.load_dataset(name, data_dir), g[f"{name}_X"]), name
+    assert g["pendigits_X"].shape[1] == 17

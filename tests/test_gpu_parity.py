@@ -290,3 +290,46 @@ def test_full_size_config2_against_fp32_and_fp64_oracle(S, O):
     assert err32 <= 1e-5 and err64 <= 1e-5
     assert np.abs(a["X"].astype(np.float64).sum(0) - g["X_sum"]).max() <= 1e-5 * len(X)
     assert drc <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["banknote", "Yeast", "Wilt", "CCPP", "pendigits"])
+def test_real_datasets_match_oracle(S, O, name):
+    """Real data sets of the reference's run_real (run_experiment.py:53-63) and
+    pendigits (17 columns), min-max normalised, eps = 0.05 (run_experiment.py:12),
+    against the committed oracle solves (tests/golden/make_real_fixtures.py)."""
+    g = golden("real_oracle")
+    X, eps = g[f"{name}_X"], float(g["eps"])
+    _say("real data set", name, X.shape, eps)
+    a = S.solve(X, eps)
+    assert a["stats"]["iterations"] == int(g[f"{name}_iterations"])
+    assert O.same_partition(a["labels"], g[f"{name}_labels"])
+    err = float(np.abs(a["X"] - g[f"{name}_final"]).max())
+    _say(f"  max |x - x_oracle| {err:.3g}")
+    if err > 1e-5:
+        # a pair at distance eps +- 1 ulp took the other branch somewhere on
+        # the way (same partition, another valid trajectory): every GPU pass
+        # must then equal the oracle's pass from the same GPU state
+        it = int(g[f"{name}_iterations"])
+        prev = X
+        for t in range(1, it + 1):
+            ga = S.iteration(prev, eps)
+            gb = O.egg_iteration(prev, eps)
+            for k in ("order", "N", "half", "border", "synced"):
+                assert np.array_equal(ga[k], gb[k]), (t, k)
+            assert np.abs(ga["X_next"] - gb["X_next"]).max() <= 1e-6, t
+            prev = S.solve(X, eps, max_iter=t)["X"]
+
+
+def test_real_harness_reads_data_files(tmp_path):
+    """experiments.run_real (run_experiment.py:547-551) on a data directory holding
+    one file in the reference's format."""
+    from egg_sync import experiments as E
+    g = golden("real_oracle")
+    X = g["banknote_X"].astype(np.float64)
+    lab = np.zeros((len(X), 1))
+    np.savetxt(tmp_path / "data_banknote_authentication.txt", np.hstack([X, lab]),
+               delimiter=",", fmt="%.9g")
+    recs = E.run_real(str(tmp_path), rounds=2)
+    assert [r["dataset"] for r in recs] == ["banknote", "banknote"]
+    assert all(r["iterations"] == int(g["banknote_iterations"]) for r in recs)
+    assert list(recs[0]["stage_times"]) == E.STAGES
