@@ -82,10 +82,31 @@ def cpu_baseline(X, eps, target_seconds=12.0):
             t0 = time.perf_counter()
             O.sync_iteration_range(X, eps, 0, targets, threads)
             secs = time.perf_counter() - t0
-    return {"value": targets / secs, "unit": "point-updates/s", "cores": threads, "kind": kind,
-            "sample": f"one SynC pass (brute-force eps-neighbourhood, SynC.cpp:352-361) for the "
-                      f"first {targets} of {n} points against all {n} points of the same data, "
-                      f"{secs:.1f} s wall"}
+        base = {"value": targets / secs, "unit": "point-updates/s", "cores": threads, "kind": kind,
+                "sample": f"one SynC pass (brute-force eps-neighbourhood, SynC.cpp:352-361) for "
+                          f"the first {targets} of {n} points against all {n} points of the same "
+                          f"data, {secs:.1f} s wall"}
+        if kind == "reference":
+            # the reference's tree-indexed CPU variant (FSynC, SynC.cpp:426-462): one
+            # thread, R-tree with node capacity 100 (python/algorithms.py:98-99)
+            try:
+                def run_tree(t):
+                    out = subprocess.run([ref, "ftime", inp, "0", str(t), "100"], check=True,
+                                         capture_output=True, text=True).stdout
+                    kv = dict(p.split("=") for p in out.split())
+                    return float(kv["build_seconds"]), float(kv["seconds"])
+                _, s = run_tree(8)
+                tt = int(min(n, max(8, 8 * 0.5 * target_seconds / max(s, 1e-3))))
+                build_s, tsecs = run_tree(tt)
+                base["tree_index"] = {
+                    "value": tt / tsecs, "unit": "point-updates/s", "cores": 1,
+                    "kind": "reference",
+                    "sample": f"FSynC's pass (R-tree range query + update, SynC.cpp:444-449) "
+                              f"for the first {tt} of {n} points, {tsecs:.1f} s wall after a "
+                              f"{build_s:.1f} s tree build over all {n} points"}
+            except Exception as e:
+                base["tree_index"] = {"error": f"{type(e).__name__}: {e}"}
+    return base
 
 
 WEIGHTS = os.path.join(ROOT, "egg-sync_amd", "egg_sync", "weights", "jump_d3.npz")

@@ -20,6 +20,13 @@
 //                      the same pass for targets [t0,t1), OpenMP-parallel like
 //                      DynamicalClustering_parallel (SynC.cpp:352-361); prints
 //                      "seconds=<wall> targets=<t1-t0> threads=<threads>"
+//   ftime <in> <t0> <t1> <B>
+//                      FSynC's pass (FDynamicalClustering, SynC.cpp:426-462): the
+//                      reference's R-tree (src/structure/RTree.h, node capacity B)
+//                      over all n points, then for targets [t0,t1) its range query,
+//                      UpdatePoint and ComputeLocationOrder, one thread as in the
+//                      reference; prints "build_seconds=<wall> seconds=<wall>
+//                      targets=<t1-t0> threads=1"
 #include <ATen/ATen.h>
 #include <omp.h>
 
@@ -29,6 +36,20 @@
 #include <vector>
 
 #include "algorithms/SynC.h"
+
+// The reference's R-tree is header-only but also defines a non-inline free
+// function, which SynC.cpp's copy already provides; a namespace of its own
+// keeps this translation unit's copy apart.
+#include <algorithm>
+#include <cmath>
+#include <iostream>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <utility>
+namespace ref_rtree {
+#include "structure/RTree.h"
+}
 
 at::Tensor synCluster_v2(float *D, int n, int d, float eps);  // SynC.cpp:88 (not in SynC.h)
 
@@ -118,6 +139,26 @@ int main(int argc, char **argv) {
         put(f, cnt.data(), cnt.size());
         put(f, rp.data(), rp.size());
         std::fclose(f);
+        return 0;
+    }
+    if (cmd == "ftime" && argc >= 6) {
+        const long t0 = std::atol(argv[3]), t1 = std::atol(argv[4]);
+        const int B = std::atoi(argv[5]);
+        std::vector<float> nxt = D;
+        auto a = std::chrono::steady_clock::now();
+        ref_rtree::RTree tree(d, B);
+        for (int p = 0; p < n; p++) tree.insert(new std::pair<int, float *>(p, &D[(size_t)p * d]));
+        auto b = std::chrono::steady_clock::now();
+        float r_local = 0.f;
+        for (long p = t0; p < t1; p++) {
+            const neighborhood N_t = tree.range(eps, &D[(size_t)p * d]);
+            UpdatePoint((int)p, N_t, D.data(), nxt.data(), n, d);
+            r_local += ComputeLocationOrder((int)p, N_t, D.data(), n, d);
+        }
+        auto c = std::chrono::steady_clock::now();
+        std::printf("build_seconds=%.6f seconds=%.6f targets=%ld threads=1 r=%g\n",
+                    std::chrono::duration<double>(b - a).count(),
+                    std::chrono::duration<double>(c - b).count(), t1 - t0, (double)r_local);
         return 0;
     }
     if (cmd == "time" && argc >= 6) {
