@@ -242,3 +242,15 @@ def test_batch_config3_slowest_instance_against_oracle(S, O):
     # the state the GPU stopped on is one the oracle's stop rule also accepts
     assert gb["flag_final"] == 1
     assert np.abs(ga["X_next"] - a["X"]).max() <= 1e-6
+
+
+def test_launch_shapes_bitwise_neutral_at_full_size(S):
+    """n = 1,000,000 (config 1): dense clusters make streamed ranges of >= 4096
+    runs, which are split into units summed in fixed point by whichever
+    wavefront takes them; the result must not depend on the launch shape."""
+    from egg_sync import data
+    X, eps = data.config1()
+    ref = S.solve(X, eps)
+    for qt, qb in ((64, 0), (128, 512)):
+        _say("config1 queue shape", qt, qb)
+        _same(ref, S.solve(X, eps, queue_threads=qt, queue_blocks=qb))
