@@ -1,7 +1,8 @@
 // Device-wide exclusive scan and stable LSD radix sort for gfx950.
 //
 // Scan: reduce-then-scan, 2048-element tiles (256 threads x 8 items), wave
-// scans with lane shuffles.  Radix sort: 8-bit digits, per-tile LDS histogram,
+// scans with lane shuffles; arrays of at most 32768 values are scanned by one
+// workgroup in a single launch.  Radix sort: 8-bit digits, per-tile LDS histogram,
 // exclusive scan of the digit-major histogram, then a stable scatter in which
 // each wavefront ranks its 64 keys by digit with 8 ballots (lanes holding the
 // same digit form a 64-bit match mask).  Stability makes the grid build
@@ -20,7 +21,9 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
-// Exclusive scan of one value per thread over the block; also returns the total.
+// Exclusive scan of one value per thread over a block of B threads; also
+// returns the total.
+template <int B = kBlock>
 __device__ __forceinline__ int block_excl_scan(int v, int *s_w, int &total) {
     const int lane = __lane_id(), wid = threadIdx.x / kWave;
     int inc = wave_incl_scan(v);
@@ -28,7 +31,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int *s_w, int &total) {
     __syncthreads();
     int woff = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / kWave; w++) {
+    for (int w = 0; w < B / kWave; w++) {
         int t = s_w[w];
         if (w < wid) woff += t;
         tot += t;
@@ -54,26 +57,37 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int *__restrict__ 
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// Single block: exclusive scan of nb block sums in chunks, total to *total.
-__global__ __launch_bounds__(kBlock) void k_scan_bsums(int *__restrict__ bsum, int nb,
-                                                       int *total, const DevState *st) {
+// One workgroup of kSingleBlock threads scans the whole array in chunks of
+// kSingleTile, carrying the running total: used for the block sums of the
+// three-kernel scan and, for arrays of at most kSingleMax values, in place of
+// it (one launch instead of three; integer sums, so the result is the same).
+// In place is allowed: each thread reads its own chunk values before it
+// writes them, and later chunks are read after earlier ones are written.
+constexpr int kSingleBlock = 1024;
+constexpr int kSingleTile = kSingleBlock * kScanItems;
+constexpr int kSingleMax = 4 * kSingleTile;
+
+__global__ __launch_bounds__(kSingleBlock) void k_scan_single(const int *in, int *out, int n,
+                                                              const int *n_dev, int *total,
+                                                              const DevState *st) {
     if (st && st->done) return;
-    __shared__ int s_w[kBlock / kWave];
+    __shared__ int s_w[kSingleBlock / kWave];
+    if (n_dev) n = min(n, *n_dev);
     int carry = 0;
-    for (int base = 0; base < nb; base += kScanTile) {
-        int idx0 = base + threadIdx.x * kScanItems;
+    for (int base = 0; base < n; base += kSingleTile) {
+        const int idx0 = base + threadIdx.x * kScanItems;
         int vals[kScanItems];
         int v = 0;
 #pragma unroll
         for (int k = 0; k < kScanItems; k++) {
-            vals[k] = (idx0 + k < nb) ? bsum[idx0 + k] : 0;
+            vals[k] = (idx0 + k < n) ? in[idx0 + k] : 0;
             v += vals[k];
         }
         int tot;
-        int ex = block_excl_scan(v, s_w, tot) + carry;
+        int ex = block_excl_scan<kSingleBlock>(v, s_w, tot) + carry;
 #pragma unroll
         for (int k = 0; k < kScanItems; k++) {
-            if (idx0 + k < nb) bsum[idx0 + k] = ex;
+            if (idx0 + k < n) out[idx0 + k] = ex;
             ex += vals[k];
         }
         carry += tot;
@@ -111,10 +125,13 @@ size_t scan_tmp_ints(int n) { return (size_t)((n + kScanTile - 1) / kScanTile) +
 
 void scan_exclusive(const int *in, int *out, int n, const int *n_dev, int *tmp, int *total,
                     const DevState *st, hipStream_t s) {
-    int nb = (n + kScanTile - 1) / kScanTile;
-    if (nb == 0) nb = 1;
+    if (n <= kSingleMax) {
+        k_scan_single<<<1, kSingleBlock, 0, s>>>(in, out, n, n_dev, total, st);
+        return;
+    }
+    const int nb = (n + kScanTile - 1) / kScanTile;
     k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, n_dev, tmp, st);
-    k_scan_bsums<<<1, kBlock, 0, s>>>(tmp, nb, total, st);
+    k_scan_single<<<1, kSingleBlock, 0, s>>>(tmp, tmp, nb, nullptr, total, st);
     k_scan_down<<<nb, kBlock, 0, s>>>(in, out, n, n_dev, tmp, st);
 }
 
