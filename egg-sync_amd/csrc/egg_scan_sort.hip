@@ -1,7 +1,7 @@
 // Device-wide exclusive scan and stable LSD radix sort for gfx950.
 //
 // Scan: reduce-then-scan, 2048-element tiles (256 threads x 8 items), wave
-// scans with lane shuffles; arrays of at most 32768 values are scanned by one
+// scans with lane shuffles; arrays of one tile or less are scanned by one
 // workgroup in a single launch.  Radix sort: 8-bit digits, per-tile LDS histogram,
 // exclusive scan of the digit-major histogram, then a stable scatter in which
 // each wavefront ranks its 64 keys by digit with 8 ballots (lanes holding the
@@ -58,14 +58,16 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int *__restrict__ 
 }
 
 // One workgroup of kSingleBlock threads scans the whole array in chunks of
-// kSingleTile, carrying the running total: used for the block sums of the
-// three-kernel scan and, for arrays of at most kSingleMax values, in place of
-// it (one launch instead of three; integer sums, so the result is the same).
-// In place is allowed: each thread reads its own chunk values before it
-// writes them, and later chunks are read after earlier ones are written.
+// kSingleTile, carrying the running total.  It scans the block sums of the
+// three-kernel scan and, in its place, arrays of at most kScanTile values (one
+// launch instead of three; integer sums, so the result is the same).  Larger
+// arrays take the three kernels: one workgroup's thread-contiguous loads and
+// stores cost ~0.85 us per thousand values on gfx950, so past a few thousand
+// values the single launch is the slower one.  In place is allowed: each
+// thread reads its own values before it writes them, and later chunks are
+// read after earlier ones are written.
 constexpr int kSingleBlock = 1024;
 constexpr int kSingleTile = kSingleBlock * kScanItems;
-constexpr int kSingleMax = 4 * kSingleTile;
 
 __global__ __launch_bounds__(kSingleBlock) void k_scan_single(const int *in, int *out, int n,
                                                               const int *n_dev, int *total,
@@ -125,7 +127,7 @@ size_t scan_tmp_ints(int n) { return (size_t)((n + kScanTile - 1) / kScanTile) +
 
 void scan_exclusive(const int *in, int *out, int n, const int *n_dev, int *tmp, int *total,
                     const DevState *st, hipStream_t s) {
-    if (n <= kSingleMax) {
+    if (n <= kScanTile) {
         k_scan_single<<<1, kSingleBlock, 0, s>>>(in, out, n, n_dev, total, st);
         return;
     }
