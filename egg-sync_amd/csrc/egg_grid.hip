@@ -424,7 +424,10 @@ __global__ __launch_bounds__(kBlock) void k_summary(
     const int lane = __lane_id();
     const int waves = gridDim.x * (blockDim.x / kWave);
     for (int wi = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; wi < W; wi += waves) {
-        int lo = 0, hi = C - 1;  // largest c with wi_off[c] <= wi
+        // largest c with wi_off[c] <= wi.  Groups are non-empty, so each has
+        // >= 1 work item: c <= wi_off[c] <= c + (W - C), hence the target lies
+        // in [wi - (W - C), wi] and the search takes log2(W - C + 1) steps.
+        int lo = max(0, wi - (W - C)), hi = min(wi, C - 1);
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (wi_off[mid] <= wi) lo = mid;
