@@ -27,7 +27,7 @@ constexpr int kScanItems = 8;     // items per thread in scans
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kSortItems = 8;     // items per thread in radix passes
 constexpr int kSortTile = kBlock * kSortItems;
-constexpr int kSumChunk = 2048;   // max points per summary work item
+constexpr int kSumChunk = 256;    // max points per summary work item
 constexpr int kTile = 64;         // max runs per update tile (one wavefront)
 constexpr int kDefaultGroup = 1;  // default cells per tile group along the last coordinate
 constexpr int kAnchorBlock = 8;   // group widths are powers of two dividing this

@@ -7,8 +7,8 @@
 // points within a cell come out in a deterministic order.
 //
 // kernel_inner_grid_stats (float atomics) becomes a deterministic wave-level
-// Kahan-compensated reduction per cell chunk (<= 2048 points), with a second
-// pass combining the chunks of large cells in order.
+// Kahan-compensated reduction per cell chunk (<= 256 points), with a second
+// pass combining the chunks of large cells (one wavefront per cell).
 #include "egg_internal.h"
 
 namespace egg {
@@ -407,6 +407,26 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(const int *__restrict__ c
         ntile[k] = (cell_run[grp_cell[k + 1]] - cell_run[grp_cell[k]] + kTile - 1) / kTile;
 }
 
+// Compensated sum of two (sum, err) pairs over the wavefront by butterflies,
+// combined in lane order so every lane ends with the same bits.
+__device__ __forceinline__ void wave_two_sum(float &s1, float &e1, float &s2, float &e2) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const float os1 = __shfl_xor(s1, o, kWave), oe1 = __shfl_xor(e1, o, kWave);
+        const float os2 = __shfl_xor(s2, o, kWave), oe2 = __shfl_xor(e2, o, kWave);
+        if (lane & o) {
+            float a1 = os1, b1 = oe1, a2 = os2, b2 = oe2;
+            two_sum_add(a1, b1, s1, e1);
+            two_sum_add(a2, b2, s2, e2);
+            s1 = a1; e1 = b1; s2 = a2; e2 = b2;
+        } else {
+            two_sum_add(s1, e1, os1, oe1);
+            two_sum_add(s2, e2, os2, oe2);
+        }
+    }
+}
+
 // One wavefront per work item (<= kSumChunk points of one cell).  Per dim:
 // S_sin = sum(sinf(x) - sinf(g*cs)), S_cos likewise (EGG_SynC.cu:362-363),
 // Kahan-compensated per lane, combined across lanes by compensated butterflies.
@@ -462,21 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_summary(
             }
             // Kahan keeps sum - comp; switch to (sum, +err) form for the butterflies
             float s1 = ss, e1 = -sc_, s2 = cs_, e2 = -cc;
-#pragma unroll
-            for (int o = kWave / 2; o > 0; o >>= 1) {
-                const float os1 = __shfl_xor(s1, o, kWave), oe1 = __shfl_xor(e1, o, kWave);
-                const float os2 = __shfl_xor(s2, o, kWave), oe2 = __shfl_xor(e2, o, kWave);
-                // combine in lane order so every lane holds the same bits
-                if (lane & o) {
-                    float a1 = os1, b1 = oe1, a2 = os2, b2 = oe2;
-                    two_sum_add(a1, b1, s1, e1);
-                    two_sum_add(a2, b2, s2, e2);
-                    s1 = a1; e1 = b1; s2 = a2; e2 = b2;
-                } else {
-                    two_sum_add(s1, e1, os1, oe1);
-                    two_sum_add(s2, e2, os2, oe2);
-                }
-            }
+            wave_two_sum(s1, e1, s2, e2);
             if (mnk) {
 #pragma unroll
                 for (int o = kWave / 2; o > 0; o >>= 1) {
@@ -504,6 +510,8 @@ __global__ __launch_bounds__(kBlock) void k_summary(
     }
 }
 
+// Cells of more than one work item: one wavefront per cell; lane k sums the
+// partials k, k + 64, ... in order, then the lanes combine by butterflies.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_summary_combine(
     const int *__restrict__ cell_start, const int *__restrict__ cell_g,
@@ -512,23 +520,29 @@ __global__ __launch_bounds__(kBlock) void k_summary_combine(
     DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int n = g.n, C = *count;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+    const int lane = __lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    for (int c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < C; c += waves) {
         const int nch = nchunk[c];
         if (nch <= 1) continue;
         const int w0 = wi_off[c];
         const float cnt = (float)(cell_start[c + 1] - cell_start[c]);
+#pragma unroll
         for (int l = 0; l < D; l++) {
             float s1 = 0.f, e1 = 0.f, s2 = 0.f, e2 = 0.f;
-            for (int k = 0; k < nch; k++) {
+            for (int k = lane; k < nch; k += kWave) {
                 const float *pp = &partial[((long long)(w0 + k) * D + l) * 4];
                 two_sum_add(s1, e1, pp[0], pp[1]);
                 two_sum_add(s2, e2, pp[2], pp[3]);
             }
-            const float a = (float)cell_g[(long long)l * n + c] * g.cs;
-            const float t1 = sinf(a) * cnt;
-            const float t2 = cosf(a) * cnt;
-            Ts[(long long)l * n + c] = t1 + (s1 + e1);
-            Tc[(long long)l * n + c] = t2 + (s2 + e2);
+            wave_two_sum(s1, e1, s2, e2);
+            if (lane == 0) {
+                const float a = (float)cell_g[(long long)l * n + c] * g.cs;
+                const float t1 = sinf(a) * cnt;
+                const float t2 = cosf(a) * cnt;
+                Ts[(long long)l * n + c] = t1 + (s1 + e1);
+                Tc[(long long)l * n + c] = t2 + (s2 + e2);
+            }
         }
     }
 }
@@ -600,11 +614,12 @@ static void group_summaries(const Geo &g, GridBufs &b, const int *starts, const 
     // work items <= groups + n / kSumChunk <= n + n / kSumChunk
     const long long max_wi = (long long)n + n / kSumChunk + 1;
     const int wb = grid_for(max_wi * kWave, kBlock, 16384);
+    const int cb = grid_for((long long)n * kWave, kBlock, 16384);  // one wavefront per group
 #define SUM_CASE(DD)                                                                          \
     case DD:                                                                                  \
         k_summary<DD><<<wb, kBlock, 0, s>>>(b.Xs, starts, coords, wi_off, nchunk, g, count,   \
                                             nwork, Ts, Tc, b.partial, mnk, mxk, st_loop);     \
-        k_summary_combine<DD><<<gb, kBlock, 0, s>>>(starts, coords, wi_off, nchunk, g, count, \
+        k_summary_combine<DD><<<cb, kBlock, 0, s>>>(starts, coords, wi_off, nchunk, g, count, \
                                                     Ts, Tc, b.partial, st_loop);              \
         break;
     switch (g.d) {
