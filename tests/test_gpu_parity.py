@@ -115,6 +115,32 @@ def test_iteration_bit_exact_late_states(S, O):
         assert np.abs(a["X_next"] - b["X_next"]).max() <= 1e-6
 
 
+def test_iteration_bit_exact_many_chunk_cells(S, O):
+    """Cells of tens of thousands of points: a cell's summary is the combine of
+    up to ~80 partial sums (more than one per lane of a wavefront)."""
+    rng = np.random.default_rng(21)
+    X = np.concatenate([np.full((20000, 2), 0.3, f32),                      # one run, one cell
+                        (0.6 + 0.03 * rng.random((15000, 2))).astype(f32),  # ~4 dense cells
+                        rng.random((5000, 2)).astype(f32)])
+    X = X[rng.permutation(len(X))]
+    _say("iteration many-chunk cells", X.shape)
+    a = S.iteration(X, 0.05)
+    b = O.egg_iteration(X, 0.05)
+    for k in ("order", "N", "half", "border", "synced"):
+        assert np.array_equal(a[k], b[k]), k
+    assert (a["flag_local"], a["flag_final"]) == (b["flag_local"], b["flag_final"])
+    # Positions against the FP64 oracle pass (equal to an FP64 brute-force
+    # update here to 1e-16).  The FP32 oracle adds the 20,000 equal terms of
+    # the bitwise-equal run one by one, as the reference does, and is ~1e-5
+    # off next to that run; the GPU weights the run's term by its length.  Bar:
+    # per point, no further from FP64 than the FP32 restatement, plus 1e-6.
+    x64 = O.egg_solve(X, 0.05, max_iter=1, fp64=True)["X"].astype(np.float64)
+    e_gpu = np.abs(a["X_next"].astype(np.float64) - x64).max(axis=1)
+    e_o32 = np.abs(b["X_next"].astype(np.float64) - x64).max(axis=1)
+    _say("  max |x - x_fp64|: gpu", e_gpu.max(), "fp32 oracle", e_o32.max())
+    assert np.all(e_gpu <= e_o32 + 1e-6), float((e_gpu - e_o32).max())
+
+
 def _cases_solve():
     from egg_sync import data
     return [("c0", *data.config0()),
