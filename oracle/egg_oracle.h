@@ -31,7 +31,14 @@
  *   eo_egg_iteration_f32   its position update against the reference's
  *                          ComputeNeighborhood+UpdatePoint (golden vectors),
  *                          against a brute-force FP64 update, and against
- *                          closed forms (two-point contraction d' = d - sin d)
+ *                          closed forms (two-point contraction d' = d - sin d);
+ *                          its stop rule (N, half, border, synced,
+ *                          EGG_SynC.cu:583-709) against a second, literal
+ *                          restatement (FP32 sqrtf compares, no thresholds),
+ *                          hand-worked edge cases and the iteration counts of
+ *                          two-point solves (tests/test_stop_rule.py); planted
+ *                          mistakes in it are caught
+ *                          (tests/test_stop_rule_mutations.py)
  *   eo_build_grid_f32      invariants (bijection, cell boxes, sortedness,
  *                          counts) and brute-force grouping
  *   eo_egg_solve_*         closed-form known answers (single point, identical
