@@ -131,7 +131,8 @@ struct DevState {
     int n_clusters;
     float lam;
     int pair_count;    // extra check: (p, q) pairs found this iteration
-    int T;             // update tiles (<= 64 points of one cell each)
+    int T;             // tiles of every cell group (<= 64 runs of one group each): extra check
+    int TU;            // tiles of the groups k_update_tile updates (sparse groups excluded)
     int tile_next;     // work-queue head of the update kernel
     int xtile_next;    // work-queue head of the extra check
     int runs;          // runs of equal positions in the sorted order

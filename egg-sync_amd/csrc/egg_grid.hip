@@ -140,17 +140,27 @@ __device__ __forceinline__ bool same_above(const uint64_t *__restrict__ skeys, i
     return eq;
 }
 
+// Sorted positions and perm, and three head flags per sorted position: cell
+// head, sub-cell head, run head (the position differs bitwise from its
+// predecessor's; equal positions always share a cell and a sub-cell).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_gather(
     const float *__restrict__ X, const int *__restrict__ perm, const uint64_t *__restrict__ skeys,
     const int *__restrict__ sidx, Geo g, float *__restrict__ Xs, int *__restrict__ perm_s,
-    int *__restrict__ head, int *__restrict__ shead, DevState *st_loop) {
+    int *__restrict__ head, int *__restrict__ shead, int *__restrict__ rhead,
+    DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int n = g.n, sb = g.sub_bits + g.hash_bits, hb = g.hash_bits;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int j = sidx ? sidx[i] : i;
+        const int jp = i > 0 ? (sidx ? sidx[i - 1] : i - 1) : j;
+        bool rh = i == 0;
 #pragma unroll
-        for (int l = 0; l < D; l++) Xs[(long long)l * n + i] = X[(long long)l * n + j];
+        for (int l = 0; l < D; l++) {
+            const float x = X[(long long)l * n + j];
+            Xs[(long long)l * n + i] = x;
+            rh |= __float_as_uint(x) != __float_as_uint(X[(long long)l * n + jp]);
+        }
         perm_s[i] = perm[j];
         int h = 1, hs = 1;
         if (i > 0) {
@@ -161,6 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(
         }
         head[i] = h;
         if (shead) shead[i] = hs;
+        rhead[i] = rh ? 1 : 0;
     }
 }
 
@@ -261,13 +272,23 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
     case DD:                                                                             \
         k_gather<DD><<<gb, kBlock, 0, s>>>(b.X, b.perm, skeys, sidx, g, b.Xs, b.perm_s,  \
                                            b.head, g.sub_bits ? b.shead : nullptr,      \
-                                           st_loop);                                    \
+                                           b.rhead, st_loop);                           \
         break;
     switch (g.d) {
         EGG_EACH_D(GATHER_CASE)
     }
 #undef GATHER_CASE
-    scan_exclusive(b.head, b.hpos, n, nullptr, b.scan_tmp, nullptr, st_loop, s);
+    {
+        // cell, run and sub-cell heads: one scan pass over the three
+        ScanSet ss{};
+        ss.in[0] = b.head;
+        ss.out[0] = b.hpos;
+        ss.in[1] = b.rhead;
+        ss.out[1] = b.rpos;
+        ss.in[2] = b.shead;
+        ss.out[2] = b.spos;
+        scan_exclusive_multi(ss, g.sub_bits ? 3 : 2, n, nullptr, b.scan_tmp, st_loop, s);
+    }
     launch_fill(b.ob, g.O, 0, st_loop, s);
     launch_fill(b.oe, g.O, 0, st_loop, s);
 #define CELLS_CASE(DD)                                                                      \
@@ -281,7 +302,6 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
 #undef CELLS_CASE
     k_outer_ranges<<<gb, kBlock, 0, s>>>(b.cell_outer, st, b.ob, b.oe, st_loop);
     if (g.sub_bits) {
-        scan_exclusive(b.shead, b.spos, n, nullptr, b.scan_tmp, nullptr, st_loop, s);
 #define SUB_CASE(DD)                                                                          \
     case DD:                                                                                  \
         k_subcells<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.head, b.shead, b.spos, b.point_cell, g,  \
@@ -309,24 +329,6 @@ __global__ __launch_bounds__(kBlock) void k_wi_count(const int *__restrict__ cel
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
         const int cnt = cell_start[c + 1] - cell_start[c];
         nchunk[c] = (cnt + kSumChunk - 1) / kSumChunk;
-    }
-}
-
-// Run heads: a sorted position starts a run when its position differs
-// bitwise from its predecessor's (equal positions always share a cell).
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_run_heads(const float *__restrict__ Xs, int n,
-                                                      int *__restrict__ head,
-                                                      DevState *st_loop) {
-    if (st_loop && st_loop->done) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        bool h = i == 0;
-#pragma unroll
-        for (int l = 0; l < D; l++)
-            if (i > 0)
-                h |= __float_as_uint(Xs[(long long)l * n + i]) !=
-                     __float_as_uint(Xs[(long long)l * n + i - 1]);
-        head[i] = h ? 1 : 0;
     }
 }
 
@@ -395,16 +397,26 @@ __global__ __launch_bounds__(kBlock) void k_group_cells(const int *__restrict__ 
     }
 }
 
-// Update tiles per group: one per 64 runs.
-__global__ __launch_bounds__(kBlock) void k_tile_count(const int *__restrict__ cell_run,
-                                                       const int *__restrict__ grp_cell,
-                                                       const DevState *st,
-                                                       int *__restrict__ ntile,
-                                                       DevState *st_loop) {
+// Per group: its tiles of 64 runs (every group: the extra check's table), the
+// same for the groups k_update_tile updates (0 for sparse groups), and the
+// runs of sparse groups (k_update_sparse), scanned together afterwards.
+__global__ __launch_bounds__(kBlock) void k_group_counts(const int *__restrict__ cell_run,
+                                                         const int *__restrict__ grp_cell,
+                                                         const DevState *st,
+                                                         int *__restrict__ ntile,
+                                                         int *__restrict__ untile,
+                                                         int *__restrict__ sparse_runs,
+                                                         DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int G = st->G;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x)
-        ntile[k] = (cell_run[grp_cell[k + 1]] - cell_run[grp_cell[k]] + kTile - 1) / kTile;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
+        const int runs = cell_run[grp_cell[k + 1]] - cell_run[grp_cell[k]];
+        const int t = (runs + kTile - 1) / kTile;
+        const bool sp = sparse_group(runs, st->runs, st->C);
+        ntile[k] = t;
+        untile[k] = sp ? 0 : t;
+        sparse_runs[k] = sp ? runs : 0;
+    }
 }
 
 // Compensated sum of two (sum, err) pairs over the wavefront by butterflies,
@@ -556,21 +568,8 @@ __global__ void k_store_total(const int *__restrict__ off, const int *__restrict
     *out = m > 0 ? off[m - 1] + cnt[m - 1] : 0;
 }
 
-// Runs of the sparse groups (sparse_group), listed for k_update_sparse: per
-// group the count (0 for the others), then the list.
-__global__ __launch_bounds__(kBlock) void k_sparse_count(const int *__restrict__ cell_run,
-                                                         const int *__restrict__ grp_cell,
-                                                         const DevState *st,
-                                                         int *__restrict__ cnt,
-                                                         DevState *st_loop) {
-    if (st_loop && st_loop->done) return;
-    const int G = st->G;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
-        const int runs = cell_run[grp_cell[k + 1]] - cell_run[grp_cell[k]];
-        cnt[k] = sparse_group(runs, st->runs, st->C) ? runs : 0;
-    }
-}
-
+// Runs of the sparse groups (sparse_group), listed for k_update_sparse at the
+// offsets of the scan of their counts.
 __global__ __launch_bounds__(kBlock) void k_sparse_list(const int *__restrict__ cell_run,
                                                         const int *__restrict__ grp_cell,
                                                         const int *__restrict__ off,
@@ -587,16 +586,16 @@ __global__ __launch_bounds__(kBlock) void k_sparse_list(const int *__restrict__ 
 }
 
 // First run of every sub-cell (a sub-cell starts where the position changes,
-// so its first sorted position heads a run; hpos holds the run-head scan).
+// so its first sorted position heads a run; rpos holds the run-head scan).
 __global__ __launch_bounds__(kBlock) void k_sub_runs(const int *__restrict__ sub_start,
-                                                     const int *__restrict__ hpos,
+                                                     const int *__restrict__ rpos,
                                                      const DevState *st,
                                                      int *__restrict__ sub_run,
                                                      DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int S = st->S;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x)
-        sub_run[k] = hpos[sub_start[k]];
+        sub_run[k] = rpos[sub_start[k]];
     if (blockIdx.x == 0 && threadIdx.x == 0) sub_run[S] = st->runs;
 }
 
@@ -633,31 +632,33 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     const int gb = grid_for(n);
     group_summaries(g, b, b.cell_start, b.cell_g, b.nchunk, b.wi_off, &st->C, &st->W, b.Ts,
                     b.Tc, nullptr, nullptr, st_loop, s);
-    // runs of equal positions and the update tiles over them
-#define RUN_CASE(DD) \
-    case DD: k_run_heads<DD><<<gb, kBlock, 0, s>>>(b.Xs, n, b.head, st_loop); break;
-    switch (g.d) {
-        EGG_EACH_D(RUN_CASE)
-    }
-#undef RUN_CASE
-    scan_exclusive(b.head, b.hpos, n, nullptr, b.scan_tmp, nullptr, st_loop, s);
-    k_runs<<<gb, kBlock, 0, s>>>(b.head, b.hpos, b.point_cell, b.cell_start, n, b.run_start,
+    // runs of equal positions (heads scanned by build_grid) and the tiles over them
+    k_runs<<<gb, kBlock, 0, s>>>(b.rhead, b.rpos, b.point_cell, b.cell_start, n, b.run_start,
                                  b.cell_run, st, st_loop);
     k_group_heads<<<gb, kBlock, 0, s>>>(b.cell_g, b.cell_outer, g, st, b.gflag, st_loop);
     scan_exclusive(b.gflag, b.gpos, n, &st->C, b.scan_tmp, &st->G, st_loop, s);
     k_group_cells<<<gb, kBlock, 0, s>>>(b.gflag, b.gpos, st, b.grp_cell, st_loop);
-    k_tile_count<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.ntile, st_loop);
-    scan_exclusive(b.ntile, b.tile_off, n, &st->G, b.scan_tmp, nullptr, st_loop, s);
-    k_store_total<<<1, 1, 0, s>>>(b.tile_off, b.ntile, &st->G, &st->T, st_loop);
+    k_group_counts<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.ntile, b.untile,
+                                         b.sparse_runs, st_loop);
+    {
+        ScanSet ss{};
+        ss.in[0] = b.ntile;
+        ss.out[0] = b.tile_off;
+        ss.total[0] = &st->T;
+        ss.in[1] = b.untile;
+        ss.out[1] = b.utile_off;
+        ss.total[1] = &st->TU;
+        ss.in[2] = b.sparse_runs;
+        ss.out[2] = b.sparse_off;
+        ss.total[2] = &st->sparse_count;
+        scan_exclusive_multi(ss, 3, n, &st->G, b.scan_tmp, st_loop, s);
+    }
     // the runs of sparse groups, for the one-lane-per-run update
-    k_sparse_count<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.sparse_runs, st_loop);
-    scan_exclusive(b.sparse_runs, b.sparse_off, n, &st->G, b.scan_tmp, &st->sparse_count,
-                   st_loop, s);
     k_sparse_list<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, b.sparse_off, st,
                                         b.sparse_runs, st_loop);
     if (g.sub_bits) {
         // sub-cell runs and totals (anchors on the half-side grid)
-        k_sub_runs<<<gb, kBlock, 0, s>>>(b.sub_start, b.hpos, st, b.sub_run, st_loop);
+        k_sub_runs<<<gb, kBlock, 0, s>>>(b.sub_start, b.rpos, st, b.sub_run, st_loop);
         Geo gf = g;
         gf.cs = g.cs * 0.5f;
         group_summaries(gf, b, b.sub_start, b.sub_g, b.snchunk, b.swi_off, &st->S, &st->WS,

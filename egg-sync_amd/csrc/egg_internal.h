@@ -12,6 +12,18 @@ size_t scan_tmp_ints(int n);
 void scan_exclusive(const int *in, int *out, int n, const int *n_dev, int *tmp, int *total,
                     const DevState *st, hipStream_t s);
 
+// Exclusive scans of nch (<= kMaxScanCh) int32 arrays of the same length in
+// one pass: channel c scans in[c] into out[c] (in place allowed) and writes
+// its sum to *total[c] if that is not null.
+constexpr int kMaxScanCh = 4;
+struct ScanSet {
+    const int *in[kMaxScanCh];
+    int *out[kMaxScanCh];
+    int *total[kMaxScanCh];
+};
+void scan_exclusive_multi(const ScanSet &ss, int nch, int n, const int *n_dev, int *tmp,
+                          const DevState *st, hipStream_t s);
+
 // Stable LSD radix sort of (key, position) pairs by key bits [0, key_bits),
 // 8 bits per pass.  keys_a/keys_b: [kw][n] u64; vals_a/vals_b: [n] int32.
 // On entry keys are in keys_a and vals are implicit (0..n-1).  Returns 0 if
@@ -41,7 +53,7 @@ struct GridBufs {
     int *vals_a, *vals_b;
     int *hist;           // [256][ntiles]
     int *scan_tmp;
-    int *head, *hpos;    // [n]
+    int *head, *hpos;    // [n] cell-head flags of the sorted positions and their scan
     int *point_cell;     // [n]
     int *cell_start;     // [n+1]
     int *cell_g;         // [d][n]
@@ -49,7 +61,8 @@ struct GridBufs {
     int *ob, *oe;        // [O]
     int *nchunk, *wi_off;// [n]
     // runs: maximal stretches of bitwise-equal consecutive positions (sorted
-    // order); head / hpos are reused for the run-head flags and their scan
+    // order); rhead / rpos are their head flags and the flags' scan
+    int *rhead, *rpos;   // [n]
     int *run_start;      // [n+1] first sorted position of each run
     int *cell_run;       // [n+1] first run of each cell
     // sub-cells (see cell_mid): shead / spos head flags and their scan,
@@ -68,7 +81,8 @@ struct GridBufs {
     // cells); a tile holds <= 64 runs of one group
     int *gflag, *gpos;   // [n] group-head flag of each cell and its exclusive scan
     int *grp_cell;       // [n+1] first cell of each group, grp_cell[G] = C
-    int *ntile, *tile_off;// [n]  tiles per group, exclusive scan
+    int *ntile, *tile_off;// [n]  tiles per group, exclusive scan (every group: extra check)
+    int *untile, *utile_off;// [n] the same for the groups of k_update_tile (not sparse)
     float *rres;         // [d][n] new position of each run
     int *sparse_runs;    // [n]   runs of sparse cell groups
     int *sparse_off;     // [n]   their offsets per group (exclusive scan)

@@ -41,23 +41,30 @@ __device__ __forceinline__ int block_excl_scan(int v, int *s_w, int &total) {
     return woff + inc - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int *__restrict__ in, int n,
-                                                        const int *n_dev, int *__restrict__ bsum,
+// Multi-channel scans: up to kMaxScanCh int32 arrays of the same length are
+// scanned by the same three launches (one pass over all of them), each
+// channel with its own block sums and total.
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(ScanSet ss, int n, const int *n_dev,
+                                                        int *__restrict__ bsum, int nb,
                                                         const DevState *st) {
     if (st && st->done) return;
-    __shared__ int s_w[kBlock / kWave];
+    __shared__ int s_w[NC][kBlock / kWave];
     if (n_dev) n = min(n, *n_dev);
     const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
-    int v = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; k++)
-        if (base + k < n) v += in[base + k];
-    int total;
-    block_excl_scan(v, s_w, total);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+    for (int c = 0; c < NC; c++) {
+        int v = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++)
+            if (base + k < n) v += ss.in[c][base + k];
+        int total;
+        block_excl_scan(v, s_w[c], total);
+        if (threadIdx.x == 0) bsum[c * nb + blockIdx.x] = total;
+    }
 }
 
-// One workgroup of kSingleBlock threads scans the whole array in chunks of
+// One workgroup of kSingleBlock threads scans whole arrays in chunks of
 // kSingleTile, carrying the running total.  It scans the block sums of the
 // three-kernel scan and, in its place, arrays of at most kScanTile values (one
 // launch instead of three; integer sums, so the result is the same).  Larger
@@ -69,72 +76,108 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int *__restrict__ 
 constexpr int kSingleBlock = 1024;
 constexpr int kSingleTile = kSingleBlock * kScanItems;
 
-__global__ __launch_bounds__(kSingleBlock) void k_scan_single(const int *in, int *out, int n,
-                                                              const int *n_dev, int *total,
+template <int NC>
+__global__ __launch_bounds__(kSingleBlock) void k_scan_single(ScanSet ss, int n, const int *n_dev,
                                                               const DevState *st) {
     if (st && st->done) return;
     __shared__ int s_w[kSingleBlock / kWave];
     if (n_dev) n = min(n, *n_dev);
-    int carry = 0;
-    for (int base = 0; base < n; base += kSingleTile) {
-        const int idx0 = base + threadIdx.x * kScanItems;
-        int vals[kScanItems];
-        int v = 0;
 #pragma unroll
-        for (int k = 0; k < kScanItems; k++) {
-            vals[k] = (idx0 + k < n) ? in[idx0 + k] : 0;
-            v += vals[k];
-        }
-        int tot;
-        int ex = block_excl_scan<kSingleBlock>(v, s_w, tot) + carry;
+    for (int c = 0; c < NC; c++) {
+        const int *in = ss.in[c];
+        int *out = ss.out[c];
+        int carry = 0;
+        for (int base = 0; base < n; base += kSingleTile) {
+            const int idx0 = base + threadIdx.x * kScanItems;
+            int vals[kScanItems];
+            int v = 0;
 #pragma unroll
-        for (int k = 0; k < kScanItems; k++) {
-            if (idx0 + k < n) out[idx0 + k] = ex;
-            ex += vals[k];
+            for (int k = 0; k < kScanItems; k++) {
+                vals[k] = (idx0 + k < n) ? in[idx0 + k] : 0;
+                v += vals[k];
+            }
+            int tot;
+            int ex = block_excl_scan<kSingleBlock>(v, s_w, tot) + carry;
+#pragma unroll
+            for (int k = 0; k < kScanItems; k++) {
+                if (idx0 + k < n) out[idx0 + k] = ex;
+                ex += vals[k];
+            }
+            carry += tot;
         }
-        carry += tot;
+        if (ss.total[c] && threadIdx.x == 0) *ss.total[c] = carry;
     }
-    if (total && threadIdx.x == 0) *total = carry;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_down(const int *in, int *out, int n,
-                                                      const int *n_dev,
-                                                      const int *__restrict__ bsum,
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_scan_down(ScanSet ss, int n, const int *n_dev,
+                                                      const int *__restrict__ bsum, int nb,
                                                       const DevState *st) {
     if (st && st->done) return;
-    __shared__ int s_w[kBlock / kWave];
+    __shared__ int s_w[NC][kBlock / kWave];
     if (n_dev) n = min(n, *n_dev);
     const long long tile0 = (long long)blockIdx.x * kScanTile;
     if (tile0 >= n) return;
     const long long base = tile0 + (long long)threadIdx.x * kScanItems;
-    int vals[kScanItems];
-    int v = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
-        vals[k] = (base + k < n) ? in[base + k] : 0;
-        v += vals[k];
-    }
-    int tot;
-    int ex = block_excl_scan(v, s_w, tot) + bsum[blockIdx.x];
+    for (int c = 0; c < NC; c++) {
+        int vals[kScanItems];
+        int v = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
-        if (base + k < n) out[base + k] = ex;
-        ex += vals[k];
+        for (int k = 0; k < kScanItems; k++) {
+            vals[k] = (base + k < n) ? ss.in[c][base + k] : 0;
+            v += vals[k];
+        }
+        int tot;
+        int ex = block_excl_scan(v, s_w[c], tot) + bsum[c * nb + blockIdx.x];
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            if (base + k < n) ss.out[c][base + k] = ex;
+            ex += vals[k];
+        }
     }
 }
 
-size_t scan_tmp_ints(int n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+size_t scan_tmp_ints(int n) {
+    return (size_t)kMaxScanCh * ((size_t)((n + kScanTile - 1) / kScanTile) + 1);
+}
 
-void scan_exclusive(const int *in, int *out, int n, const int *n_dev, int *tmp, int *total,
-                    const DevState *st, hipStream_t s) {
+template <int NC>
+static void scan_launch(const ScanSet &ss, int n, const int *n_dev, int *tmp, const DevState *st,
+                        hipStream_t s) {
     if (n <= kScanTile) {
-        k_scan_single<<<1, kSingleBlock, 0, s>>>(in, out, n, n_dev, total, st);
+        k_scan_single<NC><<<1, kSingleBlock, 0, s>>>(ss, n, n_dev, st);
         return;
     }
     const int nb = (n + kScanTile - 1) / kScanTile;
-    k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, n_dev, tmp, st);
-    k_scan_single<<<1, kSingleBlock, 0, s>>>(tmp, tmp, nb, nullptr, total, st);
-    k_scan_down<<<nb, kBlock, 0, s>>>(in, out, n, n_dev, tmp, st);
+    k_scan_reduce<NC><<<nb, kBlock, 0, s>>>(ss, n, n_dev, tmp, nb, st);
+    ScanSet bs{};
+    for (int c = 0; c < NC; c++) {
+        bs.in[c] = tmp + c * nb;
+        bs.out[c] = tmp + c * nb;
+        bs.total[c] = ss.total[c];
+    }
+    k_scan_single<NC><<<1, kSingleBlock, 0, s>>>(bs, nb, nullptr, st);
+    k_scan_down<NC><<<nb, kBlock, 0, s>>>(ss, n, n_dev, tmp, nb, st);
+}
+
+void scan_exclusive_multi(const ScanSet &ss, int nch, int n, const int *n_dev, int *tmp,
+                          const DevState *st, hipStream_t s) {
+    switch (nch) {
+        case 1: scan_launch<1>(ss, n, n_dev, tmp, st, s); break;
+        case 2: scan_launch<2>(ss, n, n_dev, tmp, st, s); break;
+        case 3: scan_launch<3>(ss, n, n_dev, tmp, st, s); break;
+        default: scan_launch<4>(ss, n, n_dev, tmp, st, s); break;
+    }
+}
+
+void scan_exclusive(const int *in, int *out, int n, const int *n_dev, int *tmp, int *total,
+                    const DevState *st, hipStream_t s) {
+    ScanSet ss{};
+    ss.in[0] = in;
+    ss.out[0] = out;
+    ss.total[0] = total;
+    scan_launch<1>(ss, n, n_dev, tmp, st, s);
 }
 
 // ------------------------------------------------------------ radix sort --

@@ -164,6 +164,8 @@ Layout carve(void *base, const Geo &g) {
     b.oe = c.take<int>((size_t)g.O);
     b.nchunk = c.take<int>(n);
     b.wi_off = c.take<int>(n);
+    b.rhead = c.take<int>(n);
+    b.rpos = c.take<int>(n);
     b.run_start = c.take<int>(n + 1);
     b.cell_run = c.take<int>(n + 1);
     b.shead = c.take<int>(n);
@@ -183,6 +185,8 @@ Layout carve(void *base, const Geo &g) {
     b.grp_cell = c.take<int>(n + 1);
     b.ntile = c.take<int>(n);
     b.tile_off = c.take<int>(n);
+    b.untile = c.take<int>(n);
+    b.utile_off = c.take<int>(n);
     b.rres = c.take<float>(n * d);
     b.sparse_runs = c.take<int>(n);
     b.sparse_off = c.take<int>(n);

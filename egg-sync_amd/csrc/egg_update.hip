@@ -241,6 +241,16 @@ __device__ __forceinline__ int next_tile(int *counter) {
     return __builtin_amdgcn_readfirstlane(t);
 }
 
+// Work claims of the queue kernels: wavefront w of the grid's W takes item w
+// first, without an atomic, and later items from the counter, offset by W.
+// A pass with fewer items than wavefronts (converged iterations) takes no
+// atomic at all; with more, each wavefront takes one per extra item.
+__device__ __forceinline__ int wave_index() {
+    return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+}
+
+__device__ __forceinline__ int wave_count() { return gridDim.x * (blockDim.x / kWave); }
+
 // Bitwise OR of v over the wavefront.
 __device__ __forceinline__ unsigned wave_or(unsigned v) {
 #pragma unroll
@@ -534,7 +544,7 @@ void k_update_tile(
     __shared__ SrcPt<D> s_src[kBlock / kWave][kWave];
     __shared__ int s_len[kBlock / kWave][kWave];
     const int n = g.n, R = g.R, od = g.od;
-    const int G = st->G, T = st->T;
+    const int G = st->G, T = st->TU;
     const float thr = g.thr_eps, thr_half = g.thr_half;
     // sub-cell classification bounds (sub_class); their own rounding keeps a
     // margin of at least 63 * 2^-24
@@ -547,12 +557,10 @@ void k_update_tile(
     int *len = s_len[threadIdx.x / kWave];
     const int *cg_od = cell_g + (long long)od * n;
     bool unsynced = false;
-    for (;;) {
-        const int t = next_tile(&st->tile_next);
-        if (t >= T) break;
+    // tiles of the non-sparse groups only (tile_off: their scan); sparse
+    // groups go to k_update_sparse
+    for (int t = wave_index(); t < T; t = wave_count() + next_tile(&st->tile_next)) {
         const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
-        if (sparse_group(cell_run[tl.gc1] - cell_run[tl.gc0], st->runs, st->C))
-            continue;  // k_update_sparse
         const int cl = tl.cl, r = tl.r;
         const bool active = tl.active;
         const int own = cell_start[cl + 1] - cell_start[cl];
@@ -808,9 +816,7 @@ __global__ __launch_bounds__(kBlock) void k_update_heavy(
     const float thr = g.thr_eps;
     // descriptors k < nd hold consecutive unit ranges in the order of k
     const int end = hdesc[nd - 1].ubase + hdesc[nd - 1].nunits;
-    for (;;) {
-        const int u = next_tile(&st->unit_next);
-        if (u >= end) break;
+    for (int u = wave_index(); u < end; u = wave_count() + next_tile(&st->unit_next)) {
         int lo = 0, hi = nd - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -1033,16 +1039,16 @@ __global__ __launch_bounds__(kBlock) void k_update_sparse(
 }
 
 // Every point takes its run's result and carries its id over to the next
-// iteration's order.  The run of sorted position i is hpos[i] + head[i] - 1.
+// iteration's order.  The run of sorted position i is rpos[i] + rhead[i] - 1.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_scatter_runs(
-    const int *__restrict__ head, const int *__restrict__ hpos, const float *__restrict__ rres,
+    const int *__restrict__ rhead, const int *__restrict__ rpos, const float *__restrict__ rres,
     const int *__restrict__ rdbg, const int *__restrict__ perm_s, float *__restrict__ Xout,
     int *__restrict__ perm_out, int n, DevState *st, DebugOut dbg) {
     if (st->done) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         perm_out[i] = perm_s[i];
-        const int r = hpos[i] + head[i] - 1;
+        const int r = rpos[i] + rhead[i] - 1;
 #pragma unroll
         for (int l = 0; l < D; l++) Xout[(long long)l * n + i] = rres[(long long)l * n + r];
         if (rdbg) {
@@ -1142,9 +1148,7 @@ __global__ __launch_bounds__(kBlock) void k_extra_tile(
     int *src_pos = s_pos[threadIdx.x / kWave];
     const int *cg_od = cell_g + (long long)od * n;
     bool hit = false;
-    for (;;) {
-        const int t = next_tile(&st->xtile_next);
-        if (t >= T) break;
+    for (int t = wave_index(); t < T; t = wave_count() + next_tile(&st->xtile_next)) {
         if (__hip_atomic_load(&st->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
         const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
         const int c0 = tl.gc0, i = tl.i;
@@ -1451,24 +1455,24 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
     case DD:                                                                                  \
         k_update_tile<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc,     \
                                             b.ob, b.oe, b.run_start, b.cell_run,              \
-                                            b.grp_cell, b.point_cell, b.tile_off,             \
+                                            b.grp_cell, b.point_cell, b.utile_off,            \
                                             b.cell_sub, b.sub_start, b.smin, b.smax,          \
                                             b.sub_run, b.TsS, b.TcS, b.rres, rdbg, b.hacc,    \
                                             b.hdesc,                                          \
                                             b.dtiles, b.rdbg, g, st);                         \
         if (b.hacc) {                                                                         \
             k_update_heavy<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_g, b.run_start, b.cell_run,     \
-                                                 b.grp_cell, b.point_cell, b.tile_off,        \
+                                                 b.grp_cell, b.point_cell, b.utile_off,       \
                                                  b.sub_run, b.hdesc, b.hacc, g, st);          \
             k_update_finish<DD><<<qb, qt, 0, s>>>(b.Xs, b.cell_start, b.cell_g, b.run_start,  \
                                                   b.cell_run, b.grp_cell, b.point_cell,       \
-                                                  b.tile_off, b.dtiles, b.rdbg, b.hacc,       \
+                                                  b.utile_off, b.dtiles, b.rdbg, b.hacc,      \
                                                   b.rres, rdbg, g, st);                       \
         }                                                                                     \
         k_update_sparse<DD><<<grid_for(g.n), kBlock, 0, s>>>(                                \
             b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.ob, b.oe, b.run_start, b.cell_run,      \
             b.point_cell, b.sparse_runs, b.rres, rdbg, g, st);                                 \
-        k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.head, b.hpos, b.rres, rdbg,     \
+        k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.rhead, b.rpos, b.rres, rdbg,   \
                                                             b.perm_s, b.X, b.perm, g.n, st,   \
                                                             d);                               \
         break;
