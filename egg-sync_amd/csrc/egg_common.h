@@ -37,6 +37,8 @@ constexpr int kHeavyRuns = 4096;  // a streamed range of at least this many runs
 constexpr int kUnitRuns = 1024;   // split into units of this many runs, any wavefront's work,
 constexpr int kHeavyCap = 65536;  // with up to this many ranges per pass in the queue,
 constexpr int kHeavyMaxD = 5;     // for d up to this (per-run fixed-point sums: (2d+2)*8 B)
+constexpr int kBlkRuns = 64;      // runs per source block of the pair update
+constexpr int kSupBlks = 64;      // source blocks per superblock
 
 // Expands X(D) for every dimension count the kernels are instantiated for;
 // host code dispatches on the runtime d with `switch (d) { EGG_EACH_D(X) }`.
@@ -44,10 +46,16 @@ constexpr int kHeavyMaxD = 5;     // for d up to this (per-run fixed-point sums:
     X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)  \
     X(17) X(18) X(19) X(20)
 
-// Whether the cell groups with `runs` runs go to the one-lane-per-run update
-// in this iteration: only when the grid is sparse overall, where streaming a
-// neighbour cell per lane is cheap.
-__host__ __device__ __forceinline__ bool sparse_group(int runs, int all_runs, int cells) {
+// Whether the cell groups with `runs` runs go to the pair update
+// (k_update_pairs) instead of the tile kernel in this iteration.  Mode 1:
+// groups of fewer than kSparseRuns runs when the grid is sparse overall, where
+// one tile per group would leave a wavefront with a lane or two; mode 2: every
+// group.
+constexpr int kPairSparse = 1;
+constexpr int kPairAll = 2;
+__host__ __device__ __forceinline__ bool sparse_group(int runs, int all_runs, int cells,
+                                                      int mode) {
+    if (mode == kPairAll) return true;
     return runs < kSparseRuns && (long long)all_runs < (long long)kSparseFactor * cells;
 }
 
@@ -78,6 +86,7 @@ struct Geo {
     int qblocks;       // workgroups of the tile-queue kernels (0 = automatic)
     int qthreads;      // threads per workgroup of those kernels (64..256, multiple of 64)
     int gw;            // update-tile groups: cells per block of the last coordinate (<= 1: one cell)
+    int pair_mode;     // which groups the pair update takes (kPairSparse / kPairAll)
 };
 
 // Sub-cells: every cell is split in two along each dimension at its centre
@@ -138,7 +147,7 @@ struct DevState {
     int runs;          // runs of equal positions in the sorted order
     int G;             // cell groups of the update tiles
     int S;             // non-empty sub-cells
-    int sparse_count;  // runs of sparse cell groups (k_update_sparse)
+    int sparse_count;  // runs of sparse cell groups (k_update_pairs)
     int WS;            // sub-cell summary work items
     double rc;         // order parameter of the last r_c evaluation
     // heavy streamed ranges of the update (k_update_heavy): descriptors
@@ -147,6 +156,7 @@ struct DevState {
     unsigned long long heavy_pack;
     int unit_next;
     int ndt;
+    int ptile_next;    // work-queue head of the pair update (k_update_pairs)
 };
 
 // ---------------------------------------------------------------- helpers --
@@ -224,6 +234,18 @@ __host__ __device__ __forceinline__ void sincos_small(float v, float &s, float &
     pc = fmaf(pc, v2, 4.16666666666667e-02f);       //  1/4!
     pc = fmaf(pc, v2, -0.5f);                       // -1/2!
     c = fmaf(v2, pc, 1.0f);
+}
+
+// sin v alone, same polynomial as sincos_small (|v| > 1: sinf).
+__host__ __device__ __forceinline__ float sin_small(float v) {
+    if (!(fabsf(v) <= 1.0f)) return sinf(v);
+    const float v2 = v * v;
+    float ps = -2.50521083854417e-08f;
+    ps = fmaf(ps, v2, 2.75573192239859e-06f);
+    ps = fmaf(ps, v2, -1.98412698412698e-04f);
+    ps = fmaf(ps, v2, 8.33333333333333e-03f);
+    ps = fmaf(ps, v2, -1.66666666666667e-01f);
+    return fmaf(v * v2, ps, v);
 }
 
 // sin v and cos v - 1 (the latter without the cancellation of cosf(v) - 1),

@@ -399,20 +399,20 @@ __global__ __launch_bounds__(kBlock) void k_group_cells(const int *__restrict__ 
 
 // Per group: its tiles of 64 runs (every group: the extra check's table), the
 // same for the groups k_update_tile updates (0 for sparse groups), and the
-// runs of sparse groups (k_update_sparse), scanned together afterwards.
+// runs of sparse groups (k_update_pairs), scanned together afterwards.
 __global__ __launch_bounds__(kBlock) void k_group_counts(const int *__restrict__ cell_run,
                                                          const int *__restrict__ grp_cell,
                                                          const DevState *st,
                                                          int *__restrict__ ntile,
                                                          int *__restrict__ untile,
                                                          int *__restrict__ sparse_runs,
-                                                         DevState *st_loop) {
+                                                         int mode, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int G = st->G;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
         const int runs = cell_run[grp_cell[k + 1]] - cell_run[grp_cell[k]];
         const int t = (runs + kTile - 1) / kTile;
-        const bool sp = sparse_group(runs, st->runs, st->C);
+        const bool sp = sparse_group(runs, st->runs, st->C, mode);
         ntile[k] = t;
         untile[k] = sp ? 0 : t;
         sparse_runs[k] = sp ? runs : 0;
@@ -568,19 +568,19 @@ __global__ void k_store_total(const int *__restrict__ off, const int *__restrict
     *out = m > 0 ? off[m - 1] + cnt[m - 1] : 0;
 }
 
-// Runs of the sparse groups (sparse_group), listed for k_update_sparse at the
+// Runs of the sparse groups (sparse_group), listed for k_update_pairs at the
 // offsets of the scan of their counts.
 __global__ __launch_bounds__(kBlock) void k_sparse_list(const int *__restrict__ cell_run,
                                                         const int *__restrict__ grp_cell,
                                                         const int *__restrict__ off,
                                                         const DevState *st,
-                                                        int *__restrict__ list,
+                                                        int *__restrict__ list, int mode,
                                                         DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int G = st->G;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
         const int r0 = cell_run[grp_cell[k]], runs = cell_run[grp_cell[k + 1]] - r0;
-        if (sparse_group(runs, st->runs, st->C))
+        if (sparse_group(runs, st->runs, st->C, mode))
             for (int j = 0; j < runs; j++) list[off[k] + j] = r0 + j;
     }
 }
@@ -639,7 +639,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     scan_exclusive(b.gflag, b.gpos, n, &st->C, b.scan_tmp, &st->G, st_loop, s);
     k_group_cells<<<gb, kBlock, 0, s>>>(b.gflag, b.gpos, st, b.grp_cell, st_loop);
     k_group_counts<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.ntile, b.untile,
-                                         b.sparse_runs, st_loop);
+                                         b.sparse_runs, g.pair_mode, st_loop);
     {
         ScanSet ss{};
         ss.in[0] = b.ntile;
@@ -653,9 +653,10 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
         ss.total[2] = &st->sparse_count;
         scan_exclusive_multi(ss, 3, n, &st->G, b.scan_tmp, st_loop, s);
     }
-    // the runs of sparse groups, for the one-lane-per-run update
-    k_sparse_list<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, b.sparse_off, st,
-                                        b.sparse_runs, st_loop);
+    // the runs of the pair update's groups (every run, in order, in mode kPairAll)
+    if (g.pair_mode != kPairAll)
+        k_sparse_list<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, b.sparse_off, st,
+                                            b.sparse_runs, g.pair_mode, st_loop);
     if (g.sub_bits) {
         // sub-cell runs and totals (anchors on the half-side grid)
         k_sub_runs<<<gb, kBlock, 0, s>>>(b.sub_start, b.rpos, st, b.sub_run, st_loop);

@@ -85,6 +85,15 @@ struct GridBufs {
     int *untile, *utile_off;// [n] the same for the groups of k_update_tile (not sparse)
     float *rres;         // [d][n] new position of each run
     int *sparse_runs;    // [n]   runs of sparse cell groups
+    // pair update (k_update_pairs): per run its cell and a flag (bit 1: a
+    // coordinate clamped into cell w-1); bounding boxes of blocks of kBlkRuns
+    // consecutive runs and of superblocks of kSupBlks blocks, and whether
+    // they hold a clamped run
+    int *rcell, *rflag;  // [n]
+    float *bmin, *bmax;  // [d][n / kBlkRuns + 1]
+    int *bclamp;         // [n / kBlkRuns + 1]
+    float *sbmin, *sbmax;// [d][n / (kBlkRuns * kSupBlks) + 1]
+    int *sbclamp;        // [n / (kBlkRuns * kSupBlks) + 1]
     int *sparse_off;     // [n]   their offsets per group (exclusive scan)
     int *rdbg;           // [4n]  N / half / border / synced of each run (diagnostics;
                          //       also the saved N / half / border of deferred runs)

@@ -189,6 +189,17 @@ Layout carve(void *base, const Geo &g) {
     b.utile_off = c.take<int>(n);
     b.rres = c.take<float>(n * d);
     b.sparse_runs = c.take<int>(n);
+    b.rcell = c.take<int>(n);
+    b.rflag = c.take<int>(n);
+    {
+        const size_t nb = n / kBlkRuns + 1, ns = n / ((size_t)kBlkRuns * kSupBlks) + 1;
+        b.bmin = c.take<float>(nb * d);
+        b.bmax = c.take<float>(nb * d);
+        b.bclamp = c.take<int>(nb);
+        b.sbmin = c.take<float>(ns * d);
+        b.sbmax = c.take<float>(ns * d);
+        b.sbclamp = c.take<int>(ns);
+    }
     b.sparse_off = c.take<int>(n);
     b.rdbg = c.take<int>(4 * n);
     b.hacc = g.d <= kHeavyMaxD ? c.take<long long>((2 * d + 2) * n) : nullptr;
@@ -333,6 +344,7 @@ void default_options(egg_sync_options &o) {
     o.queue_threads = 0;
     o.streams = 0;
     o.tile_group = 0;
+    o.pair_groups = 0;
 }
 
 void apply_tuning(const egg_sync_options &opt, Geo &g) {
@@ -342,6 +354,7 @@ void apply_tuning(const egg_sync_options &opt, Geo &g) {
     gw = std::min(gw, kAnchorBlock);
     while (gw & (gw - 1)) gw &= gw - 1;  // a power of two: groups never straddle an anchor block
     g.gw = gw;
+    g.pair_mode = opt.pair_groups == kPairAll ? kPairAll : kPairSparse;
 }
 
 template <typename T>
@@ -458,6 +471,7 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
     Workspace *ws = nullptr;
     rc = acquire(g, L, ws);
     if (rc) return rc;
+    DevState *h_state = ws->h_state;
     hipStream_t s = s_user ? s_user : ws->stream;
     if (h_D) {
         HIP_TRY(hipMemcpyAsync(L.io, h_D, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice, s));
@@ -511,9 +525,9 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
                 enqueue_iteration(g, L, opt, s, opt.timing ? &E.ev[4 * k] : nullptr);
         }
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(ws->h_state, L.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(h_state, L.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        const int real = ws->h_state->iter - done_iters;
+        const int real = h_state->iter - done_iters;
         if (opt.timing && opt.use_graph) {
             const double t = ms_between(E.ev[0], E.ev[1]);
             st_upd += t;
@@ -533,10 +547,10 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
                     so->itr_ms[done_iters + k] = (float)(b + u + x);
             }
         }
-        done_iters = ws->h_state->iter;
-        if (ws->h_state->done) break;
+        done_iters = h_state->iter;
+        if (h_state->done) break;
     }
-    const DevState loop_state = *ws->h_state;
+    const DevState loop_state = *h_state;
 
     // final grid on the converged positions and labels (EGG_SynC.cu:1504-1541)
     hipEvent_t *fe = opt.timing ? &E.ev[4 * K] : nullptr;
@@ -559,7 +573,7 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
         HIP_TRY(hipMemcpyAsync(h_final, final_dev, sizeof(float) * (size_t)n * d,
                                hipMemcpyDeviceToHost, s));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(ws->h_state, L.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_state, L.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
     const int ntr = so && so->trace_cap > 0 ? std::min(std::min(so->trace_cap, kTraceCap),
                                                         loop_state.iter) : 0;
     if (ntr > 0 && so->rc_trace)
@@ -570,7 +584,7 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
                                hipMemcpyDeviceToHost, s));
     if (fe) (void)hipEventRecord(fe[2], s);
     HIP_TRY(hipStreamSynchronize(s));
-    const DevState fin = *ws->h_state;
+    const DevState fin = *h_state;
     if (so) {
         so->iterations = loop_state.iter;
         so->status = loop_state.capped ? EGG_SYNC_CAPPED : EGG_SYNC_OK;

@@ -15,6 +15,7 @@
 // fail the reference's box test by more than a cell side, so results do not
 // change.
 #include <algorithm>
+#include <cmath>
 
 #include "egg_internal.h"
 
@@ -558,7 +559,7 @@ void k_update_tile(
     const int *cg_od = cell_g + (long long)od * n;
     bool unsynced = false;
     // tiles of the non-sparse groups only (tile_off: their scan); sparse
-    // groups go to k_update_sparse
+    // groups go to k_update_pairs
     for (int t = wave_index(); t < T; t = wave_count() + next_tile(&st->tile_next)) {
         const TileLane tl = tile_lane(t, tile_off, G, grp_cell, cell_run, run_start, point_cell);
         const int cl = tl.cl, r = tl.r;
@@ -925,117 +926,319 @@ __global__ __launch_bounds__(kBlock) void k_update_finish(
     if (__any(unsynced) && __lane_id() == 0) atomicAnd(&st->flag, 0);
 }
 
-// The update for the runs of sparse cell groups (sparse_group; the tile
-// kernel skips those groups): one lane per run, each walking its own
-// candidate cells (the Odo order), full cells through their totals, partial
-// cells run by run with sinf(x_q - x) weighted by the run's length, as
-// EGG_SynC.cu:640-703 writes it.  Nearly empty cells -- all cells of sparse
-// high-dimensional data, converged clusters -- would leave a tile's
-// wavefront with a lane or two; here every lane has a point.
+// ---------------------------------------------------------- pair update --
+//
+// The update of the runs of the cell groups sparse_group selects (the runs
+// of sparse groups, or every run), point pair by point pair instead of cell
+// by cell.  `list` holds the target runs (nullptr: runs 0 .. count-1).  In a sparse grid (high-dimensional
+// data: 250k points in 248k cells at d = 8) the neighbour cells of a point are
+// the inner cells of (2R+1)^od outer cells, almost all holding a point or
+// none, and walking them costs far more than testing the points they hold.
+// Here every run of the grid is a source: sources come in blocks of kBlkRuns
+// consecutive runs (sorted order, so spatially compact) with bounding boxes,
+// and blocks in superblocks of kSupBlks.  One wavefront takes a tile of 64
+// target runs, prunes superblocks and blocks whose box lies farther from the
+// tile's box than any counted point can, and streams the remaining blocks
+// through LDS: the loading lane stages each source run's position, the
+// bounds of its cell's box (g cs and (g+1) cs, rounded as the reference rounds
+// them) and sin v, cos v - 1 of its offset v from the tile's anchor; every
+// lane then tests every staged run against its target.
+//
+// A pair within eps, or with a clamped source (such a point lies outside its
+// cell's box), gets the reference's cell decision for the source's cell
+// (EGG_SynC.cu:612-636): a full cell adds its totals (T_s cos x - T_c sin x,
+// EGG_SynC.cu:648-663) and its count once, when the lane meets the cell's
+// first run -- every run of a full cell is within the filter radius of the
+// target (it lies in the box, whose farthest corner is within eps, and FP32
+// rounding is monotone), so that run is always met; a partial cell adds
+// sin(x_q - x) and the length of each run within eps (EGG_SynC.cu:683-696);
+// neither -> nothing.  Only cells whose outer coordinates are within R of the
+// target's are candidates, as in the reference's neighbour list
+// (EGG_SynC.cu:368-536).  Blocks are visited in increasing order, which is
+// the reference's order of outer cells, cells and points, so the sums are
+// added in the oracle's order (runs weighted by their length); N and half are
+// exactly the reference's.
+
+// Per run: its cell and flags (bit 0: the first run of its cell, bit 1: a
+// coordinate clamped into cell w-1); per block of kBlkRuns runs: the bounding
+// box of its positions and whether it holds a clamped run (one wavefront per
+// block).
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_update_sparse(
+__global__ __launch_bounds__(kBlock) void k_run_boxes(
+    const float *__restrict__ Xs, const int *__restrict__ run_start,
+    const int *__restrict__ point_cell, const int *__restrict__ cell_run, Geo g, DevState *st,
+    int *__restrict__ rcell,
+    int *__restrict__ rflag, float *__restrict__ bmin, float *__restrict__ bmax,
+    int *__restrict__ bclamp) {
+    if (st->done || st->sparse_count == 0) return;
+    const int n = g.n, runs = st->runs;
+    const int nblk = (runs + kBlkRuns - 1) / kBlkRuns, nb = n / kBlkRuns + 1;
+    const float wf = (float)g.w;
+    for (int b = wave_index(); b < nblk; b += wave_count()) {
+        const int r = b * kBlkRuns + __lane_id();
+        const bool on = r < runs;
+        float lo[D], hi[D];
+        int f = 0;
+        if (on) {
+            const int i = run_start[r];
+            const int c = point_cell[i];
+            bool clamped = false;
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                const float x = Xs[(long long)l * n + i];
+                lo[l] = hi[l] = x;
+                clamped |= x / g.cs >= wf;  // cell_coord's clamp of w to w-1
+            }
+            f = (cell_run[c] == r ? 1 : 0) | (clamped ? 2 : 0);
+            rcell[r] = c;
+            rflag[r] = f;
+        } else {
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                lo[l] = __builtin_inff();
+                hi[l] = -__builtin_inff();
+            }
+        }
+        const bool any_clamp = __any((f & 2) != 0);
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                lo[l] = fminf(lo[l], __shfl_xor(lo[l], o, kWave));
+                hi[l] = fmaxf(hi[l], __shfl_xor(hi[l], o, kWave));
+            }
+            if (__lane_id() == 0) {
+                bmin[(long long)l * nb + b] = lo[l];
+                bmax[(long long)l * nb + b] = hi[l];
+            }
+        }
+        if (__lane_id() == 0) bclamp[b] = any_clamp ? 1 : 0;
+    }
+}
+
+// Bounding box of every superblock of kSupBlks blocks, and whether it holds a
+// clamped run.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_super_boxes(
+    const float *__restrict__ bmin, const float *__restrict__ bmax,
+    const int *__restrict__ bclamp, Geo g, DevState *st, float *__restrict__ sbmin,
+    float *__restrict__ sbmax, int *__restrict__ sbclamp) {
+    if (st->done || st->sparse_count == 0) return;
+    const int n = g.n, runs = st->runs;
+    const int nblk = (runs + kBlkRuns - 1) / kBlkRuns, nb = n / kBlkRuns + 1;
+    const int nsup = (nblk + kSupBlks - 1) / kSupBlks, ns = n / (kBlkRuns * kSupBlks) + 1;
+    for (int sb = wave_index(); sb < nsup; sb += wave_count()) {
+        const int b = sb * kSupBlks + __lane_id();
+        const bool on = b < nblk;
+        const bool any_clamp = __any(on && bclamp[b] != 0);
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            float lo = on ? bmin[(long long)l * nb + b] : __builtin_inff();
+            float hi = on ? bmax[(long long)l * nb + b] : -__builtin_inff();
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                lo = fminf(lo, __shfl_xor(lo, o, kWave));
+                hi = fmaxf(hi, __shfl_xor(hi, o, kWave));
+            }
+            if (__lane_id() == 0) {
+                sbmin[(long long)l * ns + sb] = lo;
+                sbmax[(long long)l * ns + sb] = hi;
+            }
+        }
+        if (__lane_id() == 0) sbclamp[sb] = any_clamp ? 1 : 0;
+    }
+}
+
+// Whether box k of a (lo, hi) table ([D][stride]) lies within squared
+// distance thr of the box [alo, ahi] (thr_clamp if the box holds a clamped
+// run).  Box distances round like the point distances they bound; the
+// caller's thresholds carry a relative margin of 2^-10 for that.
+template <int D>
+__device__ __forceinline__ bool box_near(const float (&alo)[D], const float (&ahi)[D],
+                                         const float *__restrict__ blo,
+                                         const float *__restrict__ bhi, long long stride, int k,
+                                         float thr) {
+    float s = 0.f;
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        const float gap = fmaxf(0.f, fmaxf(blo[l * stride + k] - ahi[l], alo[l] - bhi[l * stride + k]));
+        s += gap * gap;
+    }
+    return s <= thr;
+}
+
+// A source run staged in LDS for the pair update: position, the bounds of its
+// cell's box, cell coordinates, run length, flags, cell.
+template <int D>
+struct PairSrc {
+    static constexpr int P = (D + 3) / 4 * 4;
+    float x[P], lo[P], hi[P];
+    int g[P];
+    int len, flag, cell, pad;
+};
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_update_pairs(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
-    const int *__restrict__ ob, const int *__restrict__ oe, const int *__restrict__ run_start,
-    const int *__restrict__ cell_run, const int *__restrict__ point_cell,
-    const int *__restrict__ sparse_runs, float *__restrict__ rres, int *__restrict__ rdbg,
-    Geo g, DevState *st) {
+    const int *__restrict__ run_start, const int *__restrict__ point_cell,
+    const int *__restrict__ list,
+    const int *__restrict__ rcell, const int *__restrict__ rflag,
+    const float *__restrict__ bmin, const float *__restrict__ bmax,
+    const int *__restrict__ bclamp, const float *__restrict__ sbmin,
+    const float *__restrict__ sbmax, const int *__restrict__ sbclamp, float thr_filter,
+    float thr_clamp, float *__restrict__ rres, int *__restrict__ rdbg, Geo g, DevState *st) {
     if (st->done) return;
-    const int n = g.n, od = g.od, R = g.R, count = st->sparse_count;
+    const int count = st->sparse_count;
+    if (count == 0) return;
+    __shared__ PairSrc<D> s_src[kBlock / kWave][kWave];
+    PairSrc<D> *src = s_src[threadIdx.x / kWave];
+    const int n = g.n, od = g.od, R = g.R, runs = st->runs, lane = __lane_id();
+    const int nblk = (runs + kBlkRuns - 1) / kBlkRuns, nb = n / kBlkRuns + 1;
+    const int nsup = (nblk + kSupBlks - 1) / kSupBlks, ns = n / (kBlkRuns * kSupBlks) + 1;
     const float thr = g.thr_eps, thr_half = g.thr_half;
-    const int *cg_od = cell_g + (long long)od * n;
+    const float box_thr = thr_filter * (1.0f + 0x1p-10f);
+    const float box_thr_clamp = thr_clamp * (1.0f + 0x1p-10f);
+    const int ntiles = (count + kWave - 1) / kWave;
     bool unsynced = false;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
-        const int r = sparse_runs[k];
+    for (int t = wave_index(); t < ntiles; t = wave_count() + next_tile(&st->ptile_next)) {
+        const int k = t * kWave + lane;
+        const bool active = k < count;
+        const int r = list ? list[active ? k : t * kWave] : (active ? k : t * kWave);
         const int i = run_start[r];
         const int c0 = point_cell[i];
         const int own = cell_start[c0 + 1] - cell_start[c0];
-        float x[D], sx[D], cx[D], sum[D], sml[D];
+        float x[D], sum[D], sml[D], tlo[D], thi[D];
         int g0[D];
         bool border = false;
 #pragma unroll
         for (int l = 0; l < D; l++) {
             x[l] = Xs[(long long)l * n + i];
-            g0[l] = cell_g[(long long)l * n + c0];
-            sx[l] = sinf(x[l]);
-            cx[l] = cosf(x[l]);
+            g0[l] = l < od ? cell_g[(long long)l * n + c0] : 0;
             sum[l] = 0.f;
             sml[l] = 0.f;
             border |= on_border(x[l], g.cs);
+            tlo[l] = active ? x[l] : __builtin_inff();
+            thi[l] = active ? x[l] : -__builtin_inff();
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                tlo[l] = fminf(tlo[l], __shfl_xor(tlo[l], o, kWave));
+                thi[l] = fmaxf(thi[l], __shfl_xor(thi[l], o, kWave));
+            }
         }
-        const int g0od = select_dim<D>(g0, od);
         int N = 0, half = 0;
-        Odo<D> odo;
-        odo.init(od, R);
-        while (!odo.finished) {
-            const int o = odo.outer(g0, g);
-            if (o >= 0) {
-                int cb = ob[o];
-                const int ce = oe[o];
-                if (od < D) cb = column_lower(cg_od, cb, ce, g0od - R - 1);
-                for (int c = cb; c < ce; c++) {
-                    int gc[D];
-                    bool stop;
-                    const bool in = cell_in_window<D>(cell_g, n, c, g0, od, R + 1, g0od, gc, &stop);
-                    if (stop) break;
-                    if (!in) continue;
-                    float sm, lg;
-                    box_dist<D>(gc, x, g.cs, sm, lg);
-                    if (lg <= thr) {
+        for (int sb0 = 0; sb0 < nsup; sb0 += kWave) {
+            const int sbl = sb0 + lane;
+            const bool sp = sbl < nsup &&
+                            box_near<D>(tlo, thi, sbmin, sbmax, ns, sbl,
+                                        sbclamp[sbl] ? box_thr_clamp : box_thr);
+            for (unsigned long long ms = __ballot(sp); ms; ms &= ms - 1) {
+                const int sb = sb0 + __ffsll((long long)ms) - 1;
+                const int bl = sb * kSupBlks + lane;
+                const bool bp = bl < nblk &&
+                                box_near<D>(tlo, thi, bmin, bmax, nb, bl,
+                                            bclamp[bl] ? box_thr_clamp : box_thr);
+                for (unsigned long long mb = __ballot(bp); mb; mb &= mb - 1) {
+                    const int b = sb * kSupBlks + __ffsll((long long)mb) - 1;
+                    const int rr = b * kBlkRuns + lane;
+                    if (rr < runs) {
+                        const int j = run_start[rr];
+                        const int c = rcell[rr];
+                        src[lane].len = run_start[rr + 1] - j;
+                        src[lane].flag = rflag[rr];
+                        src[lane].cell = c;
 #pragma unroll
                         for (int l = 0; l < D; l++) {
-                            const float a = Ts[(long long)l * n + c] * cx[l];
-                            const float b = Tc[(long long)l * n + c] * sx[l];
-                            const float tt = a - b;
-                            sum[l] += tt;
+                            const int gl = cell_g[(long long)l * n + c];
+                            src[lane].x[l] = Xs[(long long)l * n + j];
+                            src[lane].g[l] = gl;
+                            src[lane].lo[l] = (float)gl * g.cs;
+                            src[lane].hi[l] = (float)(gl + 1) * g.cs;
                         }
-                        const int cnt = cell_start[c + 1] - cell_start[c];
-                        N += cnt;
-                        if (border && c == c0) half += cnt;
-                        if (border && c != c0) {
-                            for (int q = cell_run[c]; q < cell_run[c + 1]; q++) {
-                                const int j = run_start[q];
-                                float xq[D];
-                                if (dist2_to<D>(x, Xs, n, j, xq) <= thr_half)
-                                    half += run_start[q + 1] - j;
+                    }
+                    wave_sync();
+                    const int m = min(kBlkRuns, runs - b * kBlkRuns);
+                    if (active) {
+                        for (int q = 0; q < m; q++) {
+                            float s = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float df = x[l] - src[q].x[l];
+                                s += df * df;
                             }
-                        }
-                    } else if (sm <= thr) {
-                        for (int q = cell_run[c]; q < cell_run[c + 1]; q++) {
-                            const int j = run_start[q];
-                            float xq[D];
-                            if (dist2_to<D>(x, Xs, n, j, xq) <= thr) {
-                                const int lq = run_start[q + 1] - j;
+                            if (!(s <= thr_filter) && !((src[q].flag & 2) && s <= thr_clamp))
+                                continue;
+                            bool win = true;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const int gq = src[q].g[l];
+                                if (l < od && (gq - g0[l] > R || g0[l] - gq > R)) win = false;
+                            }
+                            if (!win) continue;
+                            // the box test of the source's cell, EGG_SynC.cu:612-634
+                            float sm = 0.f, lg = 0.f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                float lo = src[q].lo[l] - x[l];
+                                float hi = src[q].hi[l] - x[l];
+                                lo *= lo;
+                                hi *= hi;
+                                if (lo < hi) {
+                                    sm += lo;
+                                    lg += hi;
+                                } else {
+                                    sm += hi;
+                                    lg += lo;
+                                }
+                            }
+                            const int c = src[q].cell;
+                            if (lg <= thr) {
+                                // full: totals and count once, at the cell's first run
+                                if (src[q].flag & 1) {
+                                    const int cnt = cell_start[c + 1] - cell_start[c];
+#pragma unroll
+                                    for (int l = 0; l < D; l++) {
+                                        const float a = Ts[(long long)l * n + c] * cosf(x[l]);
+                                        const float bb = Tc[(long long)l * n + c] * sinf(x[l]);
+                                        const float tt = a - bb;
+                                        sum[l] += tt;
+                                    }
+                                    N += cnt;
+                                    if (border && c == c0) half += cnt;
+                                }
+                                if (border && c != c0 && s <= thr_half) half += src[q].len;
+                            } else if (sm <= thr && s <= thr) {
+                                const int lq = src[q].len;
                                 const float w = (float)lq;  // exact: lq < 2^24
 #pragma unroll
-                                for (int l = 0; l < D; l++) sml[l] += w * sinf(xq[l] - x[l]);
+                                for (int l = 0; l < D; l++)
+                                    sml[l] += w * sin_small(src[q].x[l] - x[l]);
                                 N += lq;
                             }
                         }
                     }
+                    wave_sync();
                 }
             }
-            odo.next(od, R);
         }
-        float xo[D];
+        if (active) {
+            float xo[D];
 #pragma unroll
-        for (int l = 0; l < D; l++) {
-            float tt = sum[l] + sml[l];
-            tt = tt / (float)N;
-            xo[l] = x[l] + tt;
-        }
-        const bool synced = !((!border && N != own) || (border && N != half));
+            for (int l = 0; l < D; l++) {
+                float tt = sum[l] + sml[l];
+                tt = tt / (float)N;
+                xo[l] = x[l] + tt;
+            }
+            const bool synced = !((!border && N != own) || (border && N != half));
 #pragma unroll
-        for (int l = 0; l < D; l++) rres[(long long)l * n + r] = xo[l];
-        if (!synced) unsynced = true;
-        if (rdbg) {
-            rdbg[4 * (long long)r + 0] = N;
-            rdbg[4 * (long long)r + 1] = half;
-            rdbg[4 * (long long)r + 2] = border ? 1 : 0;
-            rdbg[4 * (long long)r + 3] = synced ? 1 : 0;
+            for (int l = 0; l < D; l++) rres[(long long)l * n + r] = xo[l];
+            if (!synced) unsynced = true;
+            if (rdbg) {
+                rdbg[4 * (long long)r + 0] = N;
+                rdbg[4 * (long long)r + 1] = half;
+                rdbg[4 * (long long)r + 2] = border ? 1 : 0;
+                rdbg[4 * (long long)r + 3] = synced ? 1 : 0;
+            }
         }
     }
-    if (__any(unsynced) && __lane_id() == 0) atomicAnd(&st->flag, 0);
+    if (__any(unsynced) && lane == 0) atomicAnd(&st->flag, 0);
 }
 
 // Every point takes its run's result and carries its id over to the next
@@ -1407,6 +1610,7 @@ __global__ void k_set_flag(DevState *st) {
     st->heavy_pack = 0;
     st->unit_next = 0;
     st->ndt = 0;
+    st->ptile_next = 0;
 }
 
 // End of an iteration (EGG_SynC.cu:1322,1425 / SynC.cpp:277).
@@ -1446,10 +1650,30 @@ static inline int queue_blocks(const Geo &g) {
     return (int)std::max<long long>(1, std::min<long long>(cap, (tiles_max + waves - 1) / waves));
 }
 
+// Filter radii of the pair update: pairs within eps, with a relative margin of
+// 2^-12 for points that cell_coord's FP32 division puts one ulp outside their
+// cell's box; for sources with a clamped coordinate (they lie up to 1 - w cs
+// outside the box of cell w-1 in each such dimension), eps + sqrt(d)(1 - w cs).
+static void pair_filters(const Geo &g, float &thr_filter, float &thr_clamp) {
+    thr_filter = g.thr_eps * (1.0f + 0x1p-12f);
+    const double gap = std::max(0.0, 1.0 - (double)g.w * (double)g.cs);
+    const double r = (double)g.eps + std::sqrt((double)g.d) * gap;
+    thr_clamp = std::max(thr_filter, (float)(r * r * (1.0 + 0x1p-10)));
+}
+
 void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *dbg,
                        hipStream_t s) {
     DebugOut d = dbg ? *dbg : DebugOut{nullptr, nullptr, nullptr, nullptr};
     const int qb = queue_blocks(g), qt = queue_threads(g);
+    const long long nblk_max = g.n / kBlkRuns + 1;
+    const int waves_per_block = kBlock / kWave;
+    const int pb = (int)std::min<long long>(16384, (nblk_max + waves_per_block - 1) / waves_per_block);
+    const int sb = (int)std::max<long long>(
+        1, std::min<long long>(1024, (nblk_max / kSupBlks + waves_per_block) / waves_per_block));
+    const int pq = (int)std::max<long long>(
+        1, std::min<long long>(kQueueBlocks, (nblk_max + waves_per_block - 1) / waves_per_block));
+    float thr_filter, thr_clamp;
+    pair_filters(g, thr_filter, thr_clamp);
     int *rdbg = (d.N || d.half || d.border || d.synced) ? b.rdbg : nullptr;
 #define UPD_CASE(DD)                                                                          \
     case DD:                                                                                  \
@@ -1469,9 +1693,16 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
                                                   b.utile_off, b.dtiles, b.rdbg, b.hacc,      \
                                                   b.rres, rdbg, g, st);                       \
         }                                                                                     \
-        k_update_sparse<DD><<<grid_for(g.n), kBlock, 0, s>>>(                                \
-            b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.ob, b.oe, b.run_start, b.cell_run,      \
-            b.point_cell, b.sparse_runs, b.rres, rdbg, g, st);                                 \
+        k_run_boxes<DD><<<pb, kBlock, 0, s>>>(b.Xs, b.run_start, b.point_cell, b.cell_run, g,   \
+                                              st, b.rcell,                                    \
+                                              b.rflag, b.bmin, b.bmax, b.bclamp);             \
+        k_super_boxes<DD><<<sb, kBlock, 0, s>>>(b.bmin, b.bmax, b.bclamp, g, st, b.sbmin,     \
+                                                b.sbmax, b.sbclamp);                          \
+        k_update_pairs<DD><<<pq, kBlock, 0, s>>>(                                             \
+            b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.run_start, b.point_cell,              \
+            g.pair_mode == kPairAll ? nullptr : b.sparse_runs, b.rcell,                       \
+            b.rflag, b.bmin, b.bmax, b.bclamp, b.sbmin, b.sbmax, b.sbclamp, thr_filter,       \
+            thr_clamp, b.rres, rdbg, g, st);                                                  \
         k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.rhead, b.rpos, b.rres, rdbg,   \
                                                             b.perm_s, b.X, b.perm, g.n, st,   \
                                                             d);                               \

@@ -36,7 +36,7 @@ class Options(ctypes.Structure):
                 ("iters_per_sync", ctypes.c_int32), ("validate", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32), ("queue_blocks", ctypes.c_int32),
                 ("queue_threads", ctypes.c_int32), ("streams", ctypes.c_int32),
-                ("tile_group", ctypes.c_int32)]
+                ("tile_group", ctypes.c_int32), ("pair_groups", ctypes.c_int32)]
 
 
 class Instance(ctypes.Structure):

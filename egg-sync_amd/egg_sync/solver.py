@@ -26,14 +26,16 @@ def _stream():
 
 def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-3, max_iter=0,
           compute_rc=False, timing=False, iters_per_sync=4, return_positions=True,
-          itr_cap=16384, use_graph=False, queue_blocks=0, queue_threads=0, tile_group=0):
+          itr_cap=16384, use_graph=False, queue_blocks=0, queue_threads=0, tile_group=0,
+          pair_groups=0):
     """Solve on the GPU.  Returns dict(labels int32 numpy, X final numpy or None, stats).
 
     use_graph replays the loop body from a captured hipGraph; queue_blocks /
     queue_threads set the launch shape of the tile-queue kernels (0 = automatic);
-    tile_group lets up to that many neighbouring cells share an update tile.
-    The first three do not change the result; tile_group changes the order of
-    FP32 additions only."""
+    tile_group lets up to that many neighbouring cells share an update tile;
+    pair_groups picks the cells updated point pair by point pair (0 automatic,
+    1 sparse groups, 2 every group).  The first three do not change the result;
+    tile_group and pair_groups change the order of FP32 additions only."""
     D = _dev_f32(X)
     n, d = D.shape
     if label_rule is None:
@@ -42,7 +44,7 @@ def solve(X, eps: float, *, stop_rule=nat.STOP_EGG, label_rule=None, lam=1 - 1e-
                               max_iter=max_iter, compute_rc=int(compute_rc), timing=int(timing),
                               iters_per_sync=iters_per_sync, use_graph=int(use_graph),
                               queue_blocks=queue_blocks, queue_threads=queue_threads,
-                              tile_group=tile_group)
+                              tile_group=tile_group, pair_groups=pair_groups)
     labels = torch.empty(n, dtype=torch.int32, device=D.device)
     final = torch.empty_like(D) if return_positions else None
     st = nat.Stats()
@@ -94,7 +96,7 @@ def pack_instances(datasets, eps_values=None):
 
 def solve_batch(D: torch.Tensor, specs, *, stop_rule=nat.STOP_EGG, label_rule=None,
                 lam=1 - 1e-3, max_iter=0, timing=False, iters_per_sync=4, use_graph=False,
-                streams=4, queue_blocks=0, queue_threads=0, tile_group=0,
+                streams=4, queue_blocks=0, queue_threads=0, tile_group=0, pair_groups=0,
                 return_positions=True):
     """Many independent solves in one call (egg_sync_solve_batch_device).
 
@@ -107,7 +109,7 @@ def solve_batch(D: torch.Tensor, specs, *, stop_rule=nat.STOP_EGG, label_rule=No
                               max_iter=max_iter, timing=int(timing),
                               iters_per_sync=iters_per_sync, use_graph=int(use_graph),
                               queue_blocks=queue_blocks, queue_threads=queue_threads,
-                              streams=streams, tile_group=tile_group)
+                              streams=streams, tile_group=tile_group, pair_groups=pair_groups)
     k = len(specs)
     inst = (nat.Instance * max(k, 1))()
     rows = fl = 0

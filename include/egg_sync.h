@@ -77,6 +77,11 @@ typedef struct egg_sync_options {
                                 this many cells (1, 2, 4 or 8; others round down),
                                 share a wavefront (0 = default).  It changes the
                                 order of FP32 additions only                      */
+    int32_t pair_groups;     /* cells updated point pair by point pair instead of
+                                by the cell walk: 0 = automatic, 1 = groups of
+                                fewer than 8 distinct positions in sparse grids,
+                                2 = every group.  It changes the order of FP32
+                                additions only                                    */
 } egg_sync_options;
 
 typedef struct egg_sync_stats {

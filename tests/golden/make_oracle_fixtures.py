@@ -9,13 +9,16 @@ with the CPU oracle without re-running it:
                       64-instance generator, n=165,583, d=2, eps=0.1; ~11,600
                       iterations, ~3 min): labels, iteration count, sampled final
                       positions.
+  c4_full_oracle.npz  config 4 (n=4,000,000, d=3, eps=0.01), run pass by pass: labels,
+                      iteration count, sampled final positions, column sums, and per
+                      pass the oracle's flags and CPU seconds.
   c2_full_oracle.npz  config 2 (n=250,000, d=8, eps=0.1; ~3 min): the same from the
                       FP32 oracle, plus the FP64 oracle's iteration count, labels
                       and sampled final positions, and the order parameter r_c of
                       the FP64 final positions computed entirely in FP64
                       (brute force over distinct positions with multiplicities).
 
-    python tests/golden/make_oracle_fixtures.py [c1] [c2]
+    python tests/golden/make_oracle_fixtures.py [c1] [c2] [c3_tail] [c4]
 """
 import os
 import sys
@@ -71,6 +74,31 @@ def c2():
     print("c2 clusters", int(r["labels"].max()) + 1, "rc64", rc64, flush=True)
 
 
+def c4():
+    import time
+    X, eps = data.config4()
+    cur, order = X, None
+    flags, secs = [], []
+    while True:
+        t0 = time.perf_counter()
+        it = O.egg_iteration(cur, eps, order)
+        secs.append(time.perf_counter() - t0)
+        flags.append((it["flag_local"], it["flag_final"]))
+        cur, order = it["X_next"], it["order"]
+        print(f"c4 pass {len(secs)}: {secs[-1]:.1f} s, flags {flags[-1]}", flush=True)
+        if it["flag_final"]:
+            break
+    labels = O.egg_labels(cur, eps)
+    rng = np.random.default_rng(0)
+    idx = np.sort(rng.choice(len(X), 40000, replace=False)).astype(np.int32)
+    np.savez_compressed(os.path.join(HERE, "c4_full_oracle.npz"), labels=labels,
+                        iterations=np.int32(len(secs)), sample_idx=idx, sample_X=cur[idx],
+                        X_sum=cur.astype(np.float64).sum(0),
+                        pass_flags=np.array(flags, np.int32),
+                        pass_cpu_s=np.array(secs, np.float64))
+    print("c4 iterations", len(secs), "clusters", int(labels.max()) + 1, flush=True)
+
+
 def c3_tail():
     X = data.config3_instances(seed=3, count=64)[27]
     r = O.egg_solve(X, 0.1)
@@ -87,4 +115,4 @@ if __name__ == "__main__":
     O.build()
     todo = sys.argv[1:] or ["c1", "c2", "c3_tail"]
     for name in todo:
-        {"c1": c1, "c2": c2, "c3_tail": c3_tail}[name]()
+        {"c1": c1, "c2": c2, "c3_tail": c3_tail, "c4": c4}[name]()
