@@ -228,6 +228,34 @@ Layout carve(void *base, const Geo &g) {
         if (rc_) return rc_;             \
     } while (0)
 
+// Workspace of egg_sync_solve_device_noalloc: a file-scope device array (each
+// GPU the library runs on holds its own copy until the process ends).  Host
+// code never uses the array's name as an address: scratch_base() takes the
+// address on the current device from hipGetSymbolAddress, once per device,
+// and every later use goes through that pointer.  Calls share the array, so
+// two calls that overlap on one GPU race; the solve initialises everything it
+// reads (loop state, error flag, every table is written before it is read).
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+void *g_scratch_ptr[64];
+
+int scratch_base(void **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return EGG_SYNC_E_NODEV;
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return e != hipSuccess ? hip_err(e) : EGG_SYNC_E_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 int acquire(const Geo &g, Layout &L, Workspace *&ws) {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
@@ -436,12 +464,21 @@ int capture_graph(hipStream_t cap, hipGraphExec_t &ge, Body &&body) {
     return 0;
 }
 
+// The solve.  With `noalloc` the workspace is the scratch array (no device
+// or pinned allocation, no runtime objects: no graph, no timing events, no
+// stream of its own -- the caller's stream or the null stream), and a
+// workload whose workspace exceeds it is rejected with EGG_SYNC_E_TOO_BIG.
 int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
-                      const egg_sync_options &opt, int32_t *d_labels, float *d_final,
+                      const egg_sync_options &opt_in, int32_t *d_labels, float *d_final,
                       egg_sync_stats *so, hipStream_t s_user, const float *h_D,
-                      int32_t *h_labels, float *h_final) {
+                      int32_t *h_labels, float *h_final, bool noalloc = false) {
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
+    egg_sync_options opt = opt_in;
+    if (noalloc) {
+        opt.use_graph = 0;
+        opt.timing = 0;
+    }
     if (opt.stop_rule != EGG_SYNC_STOP_EGG && opt.stop_rule != EGG_SYNC_STOP_RC)
         return EGG_SYNC_E_ARG;
     if (opt.label_rule != EGG_SYNC_LABELS_CELLS && opt.label_rule != EGG_SYNC_LABELS_SYNC)
@@ -469,10 +506,21 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
     const int n = g.n;
     Layout L;
     Workspace *ws = nullptr;
-    rc = acquire(g, L, ws);
-    if (rc) return rc;
-    DevState *h_state = ws->h_state;
-    hipStream_t s = s_user ? s_user : ws->stream;
+    DevState h_state_local;
+    DevState *h_state = &h_state_local;
+    hipStream_t s = s_user;
+    if (noalloc) {
+        void *base = nullptr;
+        rc = scratch_base(&base);
+        if (rc) return rc;
+        if (carve(nullptr, g).bytes > kScratchBytes) return EGG_SYNC_E_TOO_BIG;
+        L = carve(base, g);
+    } else {
+        rc = acquire(g, L, ws);
+        if (rc) return rc;
+        h_state = ws->h_state;
+        if (!s) s = ws->stream;
+    }
     if (h_D) {
         HIP_TRY(hipMemcpyAsync(L.io, h_D, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice, s));
         d_D = L.io;
@@ -1005,6 +1053,18 @@ int egg_sync_solve_device(const float *d_D, int64_t n, int32_t d, float eps,
                              (hipStream_t)stream, nullptr, nullptr, nullptr);
 }
 
+int egg_sync_solve_device_noalloc(const float *d_D, int64_t n, int32_t d, float eps,
+                                  const egg_sync_options *opt, int32_t *d_labels,
+                                  float *d_final, egg_sync_stats *stats_out, void *stream) {
+    if ((!d_D || !d_labels) && n > 0) return EGG_SYNC_E_ARG;
+    egg_sync_options o;
+    if (opt) o = *opt;
+    else default_options(o);
+    std::lock_guard<std::mutex> lk(g_mutex);
+    return solve_device_impl(d_D, n, d, eps, o, d_labels, d_final, stats_out,
+                             (hipStream_t)stream, nullptr, nullptr, nullptr, true);
+}
+
 int egg_sync_solve_batch_device(const float *d_D, const egg_sync_instance *inst,
                                 int32_t n_inst, const egg_sync_options *opt,
                                 int32_t *d_labels, float *d_final,
@@ -1156,6 +1216,8 @@ const char *egg_sync_error_string(int code) {
         case EGG_SYNC_E_ALLOC: return "device allocation failed";
         case EGG_SYNC_E_HIP: return "HIP runtime error";
         case EGG_SYNC_E_NODEV: return "no GPU device";
+        case EGG_SYNC_E_TOO_BIG:
+            return "workload too large for the no-alloc scratch array (64 MiB of workspace)";
         default: return "unknown error";
     }
 }

@@ -13,7 +13,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libeggsync.so")
 
 OK, CAPPED = 0, 1
-E_ARG, E_RANGE, E_DIM, E_KEY, E_ALLOC, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6, -7
+E_ARG, E_RANGE, E_DIM, E_KEY, E_ALLOC, E_HIP, E_NODEV, E_TOO_BIG = -1, -2, -3, -4, -5, -6, -7, -8
 STOP_EGG, STOP_RC = 0, 1
 LABELS_CELLS, LABELS_SYNC = 0, 1
 MAX_D = 20
@@ -25,7 +25,7 @@ EXPORTED = [
     "egg_sync_features_device", "egg_sync_jump_device", "egg_mlp_param_count",
     "egg_mlp_workspace_floats", "egg_mlp_forward_device", "egg_mlp_train_step_device",
     "egg_sync_solve_batch_device", "egg_sync_simple_solve_device", "egg_sync_simple_solve",
-    "egg_sync_simple_labels_device",
+    "egg_sync_simple_labels_device", "egg_sync_solve_device_noalloc",
 ]
 
 
@@ -85,6 +85,8 @@ def lib():
                                         ctypes.POINTER(Stats)]
         L.egg_sync_solve_device.argtypes = [P, i64, i32, f32, ctypes.POINTER(Options), P, P,
                                             ctypes.POINTER(Stats), P]
+        L.egg_sync_solve_device_noalloc.argtypes = [P, i64, i32, f32, ctypes.POINTER(Options),
+                                                    P, P, ctypes.POINTER(Stats), P]
         L.egg_sync_build_grid_device.argtypes = [P, i64, i32, f32, P, P, P, P, P, P, P,
                                                  ctypes.POINTER(ctypes.c_int32), P]
         L.egg_sync_iteration_device.argtypes = [P, i64, i32, f32, P, P, P, P, P, P, P,

@@ -48,6 +48,7 @@ extern "C" {
 #define EGG_SYNC_E_ALLOC (-5)       /* device allocation failed                */
 #define EGG_SYNC_E_HIP (-6)         /* HIP runtime error                       */
 #define EGG_SYNC_E_NODEV (-7)       /* no GPU visible                          */
+#define EGG_SYNC_E_TOO_BIG (-8)     /* workspace beyond the no-alloc scratch   */
 
 #define EGG_SYNC_STOP_EGG 0         /* local sync + MBR extra check (EGG_SynC.cu:706-709,1425-1489) */
 #define EGG_SYNC_STOP_RC 1          /* r_c >= lam (SynC.cpp:277)                                    */
@@ -122,6 +123,21 @@ int egg_sync_solve_ex(const float *h_D, int64_t n, int32_t d, float eps,
 int egg_sync_solve_device(const float *d_D, int64_t n, int32_t d, float eps,
                           const egg_sync_options *opt, int32_t *d_labels, float *d_final,
                           egg_sync_stats *stats_out, void *stream);
+
+/* egg_sync_solve_device without any allocation (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a 64 MiB device array inside the library, one per GPU, held
+ * until the process ends; a workload whose workspace exceeds it returns
+ * EGG_SYNC_E_TOO_BIG (about n <= 145,000 at d = 3, 90,000 at d = 8;
+ * stats_out->workspace_bytes reports the bytes a call used).  The caller
+ * passes inputs and outputs as for egg_sync_solve_device.  No graph, no
+ * timing events (use_graph and timing are ignored), the caller's stream or
+ * the null stream.  Calls share the array: two calls that overlap on one GPU
+ * race.  Results equal egg_sync_solve_device's bit for bit. */
+int egg_sync_solve_device_noalloc(const float *d_D, int64_t n, int32_t d, float eps,
+                                  const egg_sync_options *opt, int32_t *d_labels,
+                                  float *d_final, egg_sync_stats *stats_out, void *stream);
 
 /* ---- Batched solve (no counterpart in the reference: EGG_SynC solves one
  * array per call).  Many independent solves -- datasets of uneven size,
