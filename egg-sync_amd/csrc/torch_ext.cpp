@@ -84,6 +84,16 @@ std::vector<at::Tensor> DynamicalClustering_wrapper(at::Tensor &D, float eps, fl
     return run(D, eps, EGG_SYNC_STOP_RC, EGG_SYNC_LABELS_SYNC, lam, false, stream, 0);
 }
 
+// FSynC (SynC.cpp:426-487, map.cpp:21-23): the SynC iteration with R-tree range
+// queries (node capacity B) instead of a scan over all points.  The neighbour
+// sets are the same, so on the GPU it is the SynC solve above; B shaped only
+// the reference's CPU tree and is checked, not used.
+std::vector<at::Tensor> FDynamicalClustering_wrapper(at::Tensor &D, float eps, float lam,
+                                                     int64_t B, int64_t stream) {
+    if (B < 2) throw std::invalid_argument("FDynamicalClustering: node capacity B must be >= 2");
+    return DynamicalClustering_wrapper(D, eps, lam, stream);
+}
+
 std::vector<at::Tensor> GPU_DynamicalClustering_wrapper(at::Tensor &D, float eps, float lam,
                                                         int64_t stream) {
     check_input(D);
@@ -133,4 +143,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("DynamicalClustering_parallel_wrapper", &DynamicalClustering_wrapper,
           "SynC solve, r_c stop (SynC.cpp:328), on the GPU", pybind11::arg("D"),
           pybind11::arg("eps"), pybind11::arg("lam"), pybind11::arg("stream") = 0);
+    m.def("FDynamicalClustering_wrapper", &FDynamicalClustering_wrapper,
+          "FSynC solve (SynC.cpp:426), the SynC solve on the GPU", pybind11::arg("D"),
+          pybind11::arg("eps"), pybind11::arg("lam"), pybind11::arg("B"),
+          pybind11::arg("stream") = 0);
 }

@@ -5,6 +5,8 @@
     SynC_parallel(D, eps)       python/algorithms.py:94-95
     DynamicalClustering(D, eps, lam=1 - 1e-3, call=None)            :21-22
     DynamicalClustering_parallel(D, eps, lam=1 - 1e-3, call=None)   :25-26
+    FSynC(D, eps, B=100)        python/algorithms.py:98-99
+    FDynamicalClustering(D, eps, lam=1 - 1e-3, call=None, B=50)     :29-30
     simple_GPU_SynC(D, eps)     python/algorithms.py:106-107 (brute force, simple_GPU_SynC.cu)
     GPU_DynamicalClustering(D, eps, lam=1 - 1e-3, call=None)        :33-34
 
@@ -13,7 +15,10 @@ Each returns ``[labels, itr_times, stage_times, space]`` like the reference
 with values in [0, 1] (use ``data.min_max_normalize``), on the CPU (copied in,
 as the reference does) or already on the GPU (solved in place, on the current
 stream).  The reference's CPU SynC variants run here on the GPU too: same
-update, r_c stop rule and synCluster_v2 labels, different hardware.
+update, r_c stop rule and synCluster_v2 labels, different hardware.  FSynC is
+SynC with an R-tree neighbour search (SynC.cpp:426-487); the neighbour sets
+are the same, so it runs the same GPU solve (B, the tree's node capacity, is
+checked and otherwise unused).
 
 There is no CPU fallback: the calls raise if the native extension is missing.
 """
@@ -62,6 +67,16 @@ def DynamicalClustering(D: torch.Tensor, eps: float, lam: float = 1 - 1e-3, call
 
 def DynamicalClustering_parallel(D: torch.Tensor, eps: float, lam: float = 1 - 1e-3, call=None):
     return _ext().DynamicalClustering_parallel_wrapper(D, float(eps), float(lam), _stream_of(D))
+
+
+def FDynamicalClustering(D: torch.Tensor, eps: float, lam: float = 1 - 1e-3, call=None,
+                         B: int = 50):
+    """FSynC's solve (SynC.cpp:426-487): SynC with R-tree range queries; int32 labels."""
+    return _ext().FDynamicalClustering_wrapper(D, float(eps), float(lam), int(B), _stream_of(D))
+
+
+def FSynC(D: torch.Tensor, eps: float, B: int = 100):
+    return FDynamicalClustering(D, eps, lam=1 - 1e-3, B=B)
 
 
 def SynC(D: torch.Tensor, eps: float):

@@ -20,6 +20,10 @@
 //                      the same pass for targets [t0,t1), OpenMP-parallel like
 //                      DynamicalClustering_parallel (SynC.cpp:352-361); prints
 //                      "seconds=<wall> targets=<t1-t0> threads=<threads>"
+//   fsolve <in> <out> <B>
+//                      FDynamicalClustering(D, n, d, eps, lam, B) (SynC.cpp:426-487,
+//                      FSynC): labels and iteration count straight from the reference.
+//                      out: int32 n, int32 iters, int32 labels[n]
 //   ftime <in> <t0> <t1> <B>
 //                      FSynC's pass (FDynamicalClustering, SynC.cpp:426-462): the
 //                      reference's R-tree (src/structure/RTree.h, node capacity B)
@@ -138,6 +142,20 @@ int main(int argc, char **argv) {
         put(f, nxt.data(), nxt.size());
         put(f, cnt.data(), cnt.size());
         put(f, rp.data(), rp.size());
+        std::fclose(f);
+        return 0;
+    }
+    if (cmd == "fsolve" && argc >= 5) {
+        std::vector<float> D0 = D;
+        const int B = std::atoi(argv[4]);
+        std::vector<at::Tensor> res = FDynamicalClustering(D0.data(), n, d, eps, lam, B);
+        at::Tensor labels = res[0].contiguous();
+        int iters = (int)res[1].size(0);
+        FILE *f = std::fopen(argv[3], "wb");
+        if (!f) return 2;
+        put(f, &n, 1);
+        put(f, &iters, 1);
+        put(f, labels.data_ptr<int>(), (size_t)n);
         std::fclose(f);
         return 0;
     }

@@ -14,6 +14,12 @@ sources) on them, and stores inputs and outputs as .npz data:
                 neighbourhood sizes, per-point r_p.  The EGG update computes the
                 same map x -> x + mean_{N_eps} sin(x_j - x), so these pin it.
   digests.json  SHA-256 of small generator outputs (pins the generators)
+  c0_fsync.npz  config 0 through the reference's FDynamicalClustering (FSynC: the same
+                iteration with R-tree range queries, node capacity B = 100 as
+                python/algorithms.py:98 passes): labels and iteration count
+
+    python tests/golden/make_golden.py          # everything above
+    python tests/golden/make_golden.py fsync    # c0_fsync.npz only
 """
 import json
 import os
@@ -70,6 +76,25 @@ def run_step(X, eps):
     return dict(X_next=Xn.copy(), N=N.copy(), r_p=rp.copy())
 
 
+def run_fsolve(X, eps, B):
+    with tempfile.TemporaryDirectory() as td:
+        inp, out = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_input(inp, X, eps)
+        subprocess.run([REF_BIN, "fsolve", inp, out, str(B)], check=True,
+                       stdout=subprocess.DEVNULL)
+        buf = open(out, "rb").read()
+    n, iters = struct.unpack_from("<ii", buf, 0)
+    return np.frombuffer(buf, np.int32, n, 8).copy(), iters
+
+
+def fsync():
+    X0, eps0 = data.config0()
+    labels, iters = run_fsolve(X0, eps0, 100)
+    np.savez_compressed(os.path.join(HERE, "c0_fsync.npz"), X=X0, eps=np.float32(eps0),
+                        lam=LAM, B=np.int32(100), labels=labels, iterations=np.int32(iters))
+    print("c0_fsync: iterations", iters, "clusters", int(labels.max()) + 1)
+
+
 def main():
     if not os.path.exists(REF_BIN):
         sys.exit("oracle/_ref/sync_ref missing: run oracle/build_ref.sh first")
@@ -111,4 +136,10 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if not os.path.exists(REF_BIN):
+        sys.exit("oracle/_ref/sync_ref missing: run oracle/build_ref.sh first")
+    if sys.argv[1:] == ["fsync"]:
+        fsync()
+    else:
+        main()
+        fsync()

@@ -254,6 +254,25 @@ def test_torch_extension_api(S, O):
     assert itr_times.shape == (int(g["iterations"]),)
 
 
+def test_fsync_matches_reference_fsync(S):
+    """FSynC(D, eps) (python/algorithms.py:98-99) against the labels and
+    iteration count of the reference's own FDynamicalClustering on config 0
+    (tests/golden/c0_fsync.npz, written by the compiled reference)."""
+    from egg_sync import algorithms
+    g = golden("c0_fsync")
+    _say("FSynC config 0")
+    labels, itr_times, stage_times, space = algorithms.FSynC(torch.from_numpy(g["X"]),
+                                                             float(g["eps"]), B=int(g["B"]))
+    assert labels.dtype == torch.int32
+    assert np.array_equal(labels.numpy(), g["labels"])
+    assert itr_times.shape == (int(g["iterations"]),)
+    labels2, _, _, _ = algorithms.FDynamicalClustering(torch.from_numpy(g["X"]).cuda(),
+                                                       float(g["eps"]), B=50)
+    assert np.array_equal(labels2.cpu().numpy(), g["labels"])
+    with pytest.raises(Exception):
+        algorithms.FSynC(torch.from_numpy(g["X"]), float(g["eps"]), B=1)
+
+
 def test_full_size_config1_properties(S, O):
     """n = 1,000,000, d = 3, eps = 0.02: the oracle's full-size solve (fixture
     written by tests/golden/make_oracle_fixtures.py), determinism, and a

@@ -48,6 +48,16 @@ def test_sync_oracle_bitwise_against_reference(O):
     assert np.array_equal(r["r_trace"], g["r_trace"])
 
 
+def test_reference_fsync_equals_reference_sync_config0():
+    # the reference's FSynC (R-tree neighbour search, SynC.cpp:426-487) and SynC
+    # (scan, SynC.cpp:268-326) give the same labels and iteration count on
+    # config 0: the neighbour sets are equal, only their order differs
+    a, b = golden("c0_fsync"), golden("c0_sync")
+    assert np.array_equal(a["X"], b["X"]) and float(a["eps"]) == float(b["eps"])
+    assert int(a["iterations"]) == int(b["iterations"])
+    assert np.array_equal(a["labels"], b["labels"])
+
+
 @pytest.mark.parametrize("name", ["step_c0", "step_c0_mid", "step_d3", "step_d5", "step_d8"])
 def test_sync_single_pass_bitwise(O, name):
     g = golden(name)
