@@ -33,14 +33,17 @@ sys.path.insert(0, os.path.join(ROOT, "egg-sync_amd"))
 
 METRIC = "time-to-clustering (s), point-updates/s; plain vs learned form, same labels"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per update pass at n=1M (mean over the solve's 10 passes; the
-# pass is k_update_tile<3> and the kernels launched with it: k_update_heavy,
-# k_update_finish, k_update_sparse, k_scatter_runs): rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE, each divided by what the counter reads for a
-# 256 MiB copy (2x and 4x the bytes moved); see
-# profiles/update_pass_config1_n1M_counters.md.  Measured for this workload
-# only, so reported only when n is 1,000,000.
-UPDATE_TRAFFIC_BYTES_N1M = 207.5e6
+VALU_PEAK = 256 * 2.4e9  # wave64 VALU instructions/s: 256 CUs x 2.4 GHz x 1 per CU per cycle
+ITERS_PER_SYNC = 4
+# Per update pass of the n=1,000,000 workload, from rocprofv3 counter runs of
+# `bench.py --steps 1 --warmup 0 --no-cpu --forms plain` (mean over the solve's
+# passes; the pass is the kernels listed under "kernel" below): HBM bytes
+# (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a copy
+# of known size) and VALU wave instructions (SQ_INSTS_VALU).  Measured for
+# this workload only, so reported only when n is 1,000,000.
+UPDATE_PROFILE = "profiles/update_pass_config1_n1M_counters.md"
+UPDATE_TRAFFIC_BYTES_N1M = 223.65e6
+UPDATE_VALU_INSTS_N1M = 4.25e9
 
 
 def cpu_baseline(X, eps, target_seconds=12.0):
@@ -212,6 +215,39 @@ def copy_bandwidth(nbytes=1 << 30, reps=5):
     return gbs
 
 
+def resident_solver(D, eps, iters_per_sync):
+    """Device-resident solve through the C ABI (egg_sync_solve_device): input and
+    labels stay in HBM, only the loop state (a few dozen bytes per batch of
+    iterations) crosses to the host."""
+    import ctypes
+    import numpy as np
+    import torch
+    from egg_sync import _native as nat
+    n, d = D.shape
+    labels = torch.empty(n, dtype=torch.int32, device=D.device)
+    itr = np.zeros(4096, np.float32)
+    cells = np.zeros(4096, np.int32)
+
+    def run(timing):
+        opt = nat.default_options(timing=int(timing), iters_per_sync=iters_per_sync)
+        st = nat.Stats()
+        st.itr_ms = itr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        st.itr_ms_cap = len(itr)
+        st.trace_cap = len(cells)
+        st.cells_trace = cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        nat.check(nat.lib().egg_sync_solve_device(nat.ptr(D), n, d, float(eps),
+                                                  ctypes.byref(opt), nat.ptr(labels), None,
+                                                  ctypes.byref(st), stream),
+                  "egg_sync_solve_device")
+        it = st.iterations
+        return {"iterations": it, "n_clusters": st.n_clusters,
+                "update_kernel_ms": st.update_kernel_ms, "update_launches": st.update_launches,
+                "stage_ms": list(st.stage_ms), "itr_ms": itr[:it].copy() if timing else None,
+                "cells_trace": cells[:min(it, len(cells))].copy()}
+    return run, labels
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,7 +255,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--iters-per-sync", type=int, default=8)
+    ap.add_argument("--iters-per-sync", type=int, default=ITERS_PER_SYNC)
     ap.add_argument("--forms", default="plain,plain_d8,learned,batched",
                     help="comma list; plain is always measured")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -234,8 +270,7 @@ def main():
 
     import numpy as np
     import torch
-    from egg_sync import _native as nat
-    from egg_sync import data, solver
+    from egg_sync import algorithms, data
 
     if args.copy_calibration:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -248,34 +283,43 @@ def main():
     n, d = X.shape
     D = torch.from_numpy(X).cuda()
     torch.cuda.synchronize()
-
-    def one(timing):
-        return solver.solve(D, eps, timing=timing, iters_per_sync=args.iters_per_sync,
-                            return_positions=False)
+    run, labels_dev = resident_solver(D, eps, args.iters_per_sync)
 
     for _ in range(args.warmup):
-        one(True)
+        run(True)
     torch.cuda.synchronize()
     results = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        results.append(one(True))
+        results.append(run(True))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    labels0 = labels_dev.cpu().numpy()
 
-    iters = [r["stats"]["iterations"] for r in results]
-    labels0 = results[0]["labels"]
-    same = all(np.array_equal(r["labels"], labels0) for r in results)
+    iters = [r["iterations"] for r in results]
     updates = sum(n * it for it in iters)
     value = updates / wall
-    # roofline of the update kernel over the timed launches
-    upd_ms = sum(r["stats"]["update_kernel_ms"] for r in results)
-    launches = sum(r["stats"]["update_launches"] for r in results)
-    cells = np.concatenate([r["stats"]["cells_trace"] for r in results]).astype(np.float64)
+    # roofline of the update pass over the timed launches (HIP events on the
+    # solver's stream around every pass)
+    upd_ms = sum(r["update_kernel_ms"] for r in results)
+    launches = sum(r["update_launches"] for r in results)
+    cells = np.concatenate([r["cells_trace"] for r in results]).astype(np.float64)
     bytes_per_launch = float(np.mean(8.0 * n * d + (12.0 + 8.0 * d) * cells))
     avg_launch_s = upd_ms / 1000.0 / max(launches, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    st = results[-1]["stats"]
+    st = results[-1]
+
+    # the reference's own call, host tensor in and host labels out
+    # (run_experiment.py:321-325 times EGG_SynC(D_cpu, eps) this way)
+    Xc = torch.from_numpy(X)
+    algorithms.EGG_SynC(Xc, eps)
+    host_s = []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        lab_h, _, _, _ = algorithms.EGG_SynC(Xc, eps)
+        host_s.append(time.perf_counter() - t1)
+    same_host = bool(np.array_equal(lab_h.numpy() >= 0, labels0 >= 0))
+    full_size = n == 1_000_000
     line = {
         "metric": METRIC,
         "value": value,
@@ -290,24 +334,41 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded 20-blob Gaussian mixture (Zipf 1.1 sizes, per-axis sigma "
                 "U[0.005,0.02]) + 2% uniform noise, min-max normalised to [0,1]^3",
-        "config": {"workload": f"plain EGG-SynC solve, n={n}, d={d}, eps={eps}, inputs resident "
-                               "in HBM, labels left in HBM",
+        "config": {"workload": f"plain EGG-SynC solve, n={n}, d={d}, eps={eps}; a step is one "
+                               "solve from the input in HBM to the labels in HBM "
+                               "(egg_sync_solve_device)",
                    "form": "plain", "n": n, "d": d, "eps": eps, "iterations": iters[0],
+                   "iters_per_sync": args.iters_per_sync,
                    "time_to_clustering_s": wall / args.steps,
-                   "clusters": st["n_clusters"], "labels_identical_across_steps": same,
+                   "time_to_clustering_host_s": float(np.median(host_s)),
+                   "host_call": "EGG_SynC(D_cpu, eps) of the drop-in API: host tensor in, host "
+                                "labels out, 12 MB copied each way (run_experiment.py:321-325)",
+                   "host_point_updates_per_s": n * iters[0] / float(np.median(host_s)),
+                   "host_noise_mask_equal": same_host,
+                   "clusters": st["n_clusters"],
                    "stage_ms_last_step": [round(x, 3) for x in st["stage_ms"]],
-                   "iteration_ms_last_step":
-                       [round(float(x), 3) for x in (st["itr_ms"] if st["itr_ms"] is not None else [])],
+                   "iteration_ms_last_step": [round(float(x), 3) for x in st["itr_ms"]],
                    "cells_per_iteration_last_step": [int(c) for c in st["cells_trace"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": UPDATE_TRAFFIC_BYTES_N1M if n == 1_000_000 else None,
+                     "traffic": UPDATE_TRAFFIC_BYTES_N1M if full_size else None,
+                     "traffic_source": UPDATE_PROFILE if full_size else None,
                      "kernel": "update pass: k_update_tile<3> + k_update_heavy<3> + "
-                               "k_update_finish<3> + k_update_sparse<3> + k_scatter_runs<3>",
+                               "k_update_finish<3> + k_run_boxes<3> + k_super_boxes<3> + "
+                               "k_update_pairs<3> + k_scatter_runs<3>",
                      "launches": launches,
                      "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
+    if full_size:
+        # the pass is bound by VALU instruction issue, not by HBM bytes: wave
+        # instructions per pass (rocprofv3 SQ_INSTS_VALU over the same solve)
+        # over the pass time measured here, against 256 CUs x 2.4 GHz x one
+        # wave64 VALU instruction per CU per cycle
+        valu = UPDATE_VALU_INSTS_N1M / avg_launch_s
+        line["roofline"]["valu"] = {"achieved": valu, "peak": VALU_PEAK, "unit": "wave-instr/s",
+                                    "frac": valu / VALU_PEAK, "instructions_per_launch":
+                                    UPDATE_VALU_INSTS_N1M, "source": UPDATE_PROFILE}
     try:
         cp = copy_bandwidth()
         line["roofline"]["copy_gbs_measured"] = cp

@@ -1065,17 +1065,38 @@ __device__ __forceinline__ bool box_near(const float (&alo)[D], const float (&ah
     return s <= thr;
 }
 
-// A source run staged in LDS for the pair update: position, the bounds of its
-// cell's box, cell coordinates, run length, flags, cell.
+// Source runs of a block staged in LDS for the pair update.  Positions are
+// kept twice: row-major for the distance tests, where every lane reads the
+// same run (broadcast, 16-byte loads), and with the cell box bounds, cell
+// coordinates, length, flags and cell in columns for the accepted pairs,
+// where each lane reads a run of its own (consecutive runs in consecutive
+// banks).
 template <int D>
-struct PairSrc {
+struct PairStage {
     static constexpr int P = (D + 3) / 4 * 4;
-    float x[P], lo[P], hi[P];
-    int g[P];
-    int len, flag, cell, pad;
+    float xr[kBlkRuns][P];
+    float x[D][kBlkRuns], lo[D][kBlkRuns], hi[D][kBlkRuns];
+    int g[D][kBlkRuns];
+    int len[kBlkRuns], flag[kBlkRuns], cell[kBlkRuns];
 };
 
-template <int D>
+// sin v: the polynomial of sin_small for |v| <= 1 when the caller knows that
+// bound holds (SMALL), else sinf.
+template <bool SMALL>
+__device__ __forceinline__ float sin_pair(float v) {
+    if (!SMALL) return sinf(v);
+    const float v2 = v * v;
+    float ps = -2.50521083854417e-08f;
+    ps = fmaf(ps, v2, 2.75573192239859e-06f);
+    ps = fmaf(ps, v2, -1.98412698412698e-04f);
+    ps = fmaf(ps, v2, 8.33333333333333e-03f);
+    ps = fmaf(ps, v2, -1.66666666666667e-01f);
+    return fmaf(v * v2, ps, v);
+}
+
+// SMALL: every pair the filter passes is closer than 1 in each coordinate
+// (filter radius below 1), so sin(x_q - x) takes the polynomial.
+template <int D, bool SMALL>
 __global__ __launch_bounds__(kBlock) void k_update_pairs(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
@@ -1089,8 +1110,8 @@ __global__ __launch_bounds__(kBlock) void k_update_pairs(
     if (st->done) return;
     const int count = st->sparse_count;
     if (count == 0) return;
-    __shared__ PairSrc<D> s_src[kBlock / kWave][kWave];
-    PairSrc<D> *src = s_src[threadIdx.x / kWave];
+    __shared__ PairStage<D> s_stage[kBlock / kWave];
+    PairStage<D> &S = s_stage[threadIdx.x / kWave];
     const int n = g.n, od = g.od, R = g.R, runs = st->runs, lane = __lane_id();
     const int nblk = (runs + kBlkRuns - 1) / kBlkRuns, nb = n / kBlkRuns + 1;
     const int nsup = (nblk + kSupBlks - 1) / kSupBlks, ns = n / (kBlkRuns * kSupBlks) + 1;
@@ -1141,76 +1162,96 @@ __global__ __launch_bounds__(kBlock) void k_update_pairs(
                     if (rr < runs) {
                         const int j = run_start[rr];
                         const int c = rcell[rr];
-                        src[lane].len = run_start[rr + 1] - j;
-                        src[lane].flag = rflag[rr];
-                        src[lane].cell = c;
+                        S.len[lane] = run_start[rr + 1] - j;
+                        S.flag[lane] = rflag[rr];
+                        S.cell[lane] = c;
 #pragma unroll
                         for (int l = 0; l < D; l++) {
                             const int gl = cell_g[(long long)l * n + c];
-                            src[lane].x[l] = Xs[(long long)l * n + j];
-                            src[lane].g[l] = gl;
-                            src[lane].lo[l] = (float)gl * g.cs;
-                            src[lane].hi[l] = (float)(gl + 1) * g.cs;
+                            const float xq = Xs[(long long)l * n + j];
+                            S.xr[lane][l] = xq;
+                            S.x[l][lane] = xq;
+                            S.g[l][lane] = gl;
+                            S.lo[l][lane] = (float)gl * g.cs;
+                            S.hi[l][lane] = (float)(gl + 1) * g.cs;
                         }
                     }
                     wave_sync();
                     const int m = min(kBlkRuns, runs - b * kBlkRuns);
+                    // the staged runs this lane must look at: within the
+                    // filter radius, or clamped and within the wider one
+                    unsigned long long acc = 0;
                     if (active) {
                         for (int q = 0; q < m; q++) {
                             float s = 0.f;
 #pragma unroll
                             for (int l = 0; l < D; l++) {
-                                const float df = x[l] - src[q].x[l];
+                                const float df = x[l] - S.xr[q][l];
                                 s += df * df;
                             }
-                            if (!(s <= thr_filter) && !((src[q].flag & 2) && s <= thr_clamp))
-                                continue;
+                            const bool take = s <= thr_filter ||
+                                              ((S.flag[q] & 2) && s <= thr_clamp);
+                            acc |= take ? (1ull << q) : 0ull;
+                        }
+                    }
+                    // each lane walks its own accepted runs, lowest first (the
+                    // block's order, hence the oracle's)
+                    while (__any(acc != 0)) {
+                        if (acc) {
+                            const int q = __ffsll((long long)acc) - 1;
+                            acc &= acc - 1;
+                            float s = 0.f;
                             bool win = true;
 #pragma unroll
                             for (int l = 0; l < D; l++) {
-                                const int gq = src[q].g[l];
-                                if (l < od && (gq - g0[l] > R || g0[l] - gq > R)) win = false;
-                            }
-                            if (!win) continue;
-                            // the box test of the source's cell, EGG_SynC.cu:612-634
-                            float sm = 0.f, lg = 0.f;
-#pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                float lo = src[q].lo[l] - x[l];
-                                float hi = src[q].hi[l] - x[l];
-                                lo *= lo;
-                                hi *= hi;
-                                if (lo < hi) {
-                                    sm += lo;
-                                    lg += hi;
-                                } else {
-                                    sm += hi;
-                                    lg += lo;
+                                const float df = x[l] - S.x[l][q];
+                                s += df * df;
+                                if (l < od) {
+                                    const int gq = S.g[l][q];
+                                    if (gq - g0[l] > R || g0[l] - gq > R) win = false;
                                 }
                             }
-                            const int c = src[q].cell;
-                            if (lg <= thr) {
-                                // full: totals and count once, at the cell's first run
-                                if (src[q].flag & 1) {
-                                    const int cnt = cell_start[c + 1] - cell_start[c];
+                            if (win) {
+                                // the box test of the source's cell, EGG_SynC.cu:612-634
+                                float sm = 0.f, lg = 0.f;
 #pragma unroll
-                                    for (int l = 0; l < D; l++) {
-                                        const float a = Ts[(long long)l * n + c] * cosf(x[l]);
-                                        const float bb = Tc[(long long)l * n + c] * sinf(x[l]);
-                                        const float tt = a - bb;
-                                        sum[l] += tt;
+                                for (int l = 0; l < D; l++) {
+                                    float lo = S.lo[l][q] - x[l];
+                                    float hi = S.hi[l][q] - x[l];
+                                    lo *= lo;
+                                    hi *= hi;
+                                    if (lo < hi) {
+                                        sm += lo;
+                                        lg += hi;
+                                    } else {
+                                        sm += hi;
+                                        lg += lo;
                                     }
-                                    N += cnt;
-                                    if (border && c == c0) half += cnt;
                                 }
-                                if (border && c != c0 && s <= thr_half) half += src[q].len;
-                            } else if (sm <= thr && s <= thr) {
-                                const int lq = src[q].len;
-                                const float w = (float)lq;  // exact: lq < 2^24
+                                const int c = S.cell[q];
+                                if (lg <= thr) {
+                                    // full: totals and count once, at the cell's first run
+                                    if (S.flag[q] & 1) {
+                                        const int cnt = cell_start[c + 1] - cell_start[c];
 #pragma unroll
-                                for (int l = 0; l < D; l++)
-                                    sml[l] += w * sin_small(src[q].x[l] - x[l]);
-                                N += lq;
+                                        for (int l = 0; l < D; l++) {
+                                            const float a = Ts[(long long)l * n + c] * cosf(x[l]);
+                                            const float bb = Tc[(long long)l * n + c] * sinf(x[l]);
+                                            const float tt = a - bb;
+                                            sum[l] += tt;
+                                        }
+                                        N += cnt;
+                                        if (border && c == c0) half += cnt;
+                                    }
+                                    if (border && c != c0 && s <= thr_half) half += S.len[q];
+                                } else if (sm <= thr && s <= thr) {
+                                    const int lq = S.len[q];
+                                    const float w = (float)lq;  // exact: lq < 2^24
+#pragma unroll
+                                    for (int l = 0; l < D; l++)
+                                        sml[l] += w * sin_pair<SMALL>(S.x[l][q] - x[l]);
+                                    N += lq;
+                                }
                             }
                         }
                     }
@@ -1674,6 +1715,12 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
         1, std::min<long long>(kQueueBlocks, (nblk_max + waves_per_block - 1) / waves_per_block));
     float thr_filter, thr_clamp;
     pair_filters(g, thr_filter, thr_clamp);
+    // every filtered pair is closer than 1 in each coordinate
+    const bool pair_small = thr_clamp < 1.0f;
+#define PAIR_ARGS                                                                               \
+    b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.run_start, b.point_cell,                        \
+        g.pair_mode == kPairAll ? nullptr : b.sparse_runs, b.rcell, b.rflag, b.bmin, b.bmax,    \
+        b.bclamp, b.sbmin, b.sbmax, b.sbclamp, thr_filter, thr_clamp, b.rres, rdbg, g, st
     int *rdbg = (d.N || d.half || d.border || d.synced) ? b.rdbg : nullptr;
 #define UPD_CASE(DD)                                                                          \
     case DD:                                                                                  \
@@ -1698,11 +1745,10 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
                                               b.rflag, b.bmin, b.bmax, b.bclamp);             \
         k_super_boxes<DD><<<sb, kBlock, 0, s>>>(b.bmin, b.bmax, b.bclamp, g, st, b.sbmin,     \
                                                 b.sbmax, b.sbclamp);                          \
-        k_update_pairs<DD><<<pq, kBlock, 0, s>>>(                                             \
-            b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.run_start, b.point_cell,              \
-            g.pair_mode == kPairAll ? nullptr : b.sparse_runs, b.rcell,                       \
-            b.rflag, b.bmin, b.bmax, b.bclamp, b.sbmin, b.sbmax, b.sbclamp, thr_filter,       \
-            thr_clamp, b.rres, rdbg, g, st);                                                  \
+        if (pair_small)                                                                       \
+            k_update_pairs<DD, true><<<pq, kBlock, 0, s>>>(PAIR_ARGS);                        \
+        else                                                                                  \
+            k_update_pairs<DD, false><<<pq, kBlock, 0, s>>>(PAIR_ARGS);                       \
         k_scatter_runs<DD><<<grid_for(g.n), kBlock, 0, s>>>(b.rhead, b.rpos, b.rres, rdbg,   \
                                                             b.perm_s, b.X, b.perm, g.n, st,   \
                                                             d);                               \
@@ -1711,6 +1757,7 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
         EGG_EACH_D(UPD_CASE)
     }
 #undef UPD_CASE
+#undef PAIR_ARGS
 }
 
 void launch_extra_check(const Geo &g, GridBufs &b, DevState *st, hipStream_t s) {
