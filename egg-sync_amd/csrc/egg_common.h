@@ -47,16 +47,23 @@ constexpr int kSupBlks = 64;      // source blocks per superblock
     X(17) X(18) X(19) X(20)
 
 // Whether the cell groups with `runs` runs go to the pair update
-// (k_update_pairs) instead of the tile kernel in this iteration.  Mode 1:
-// groups of fewer than kSparseRuns runs when the grid is sparse overall, where
-// one tile per group would leave a wavefront with a lane or two; mode 2: every
-// group.
+// (k_update_pairs) instead of the tile kernel in this iteration.  Mode
+// kPairSparse: groups of fewer than kSparseRuns runs when the grid is sparse
+// overall (fewer than kSparseFactor runs per cell), where one tile per group
+// would leave a wavefront with a lane or two; kPairAll: every group;
+// kPairAuto: every group of a sparse grid from kPairAllMinD dimensions on
+// (the tile kernel walks (2R+1)^od outer cells per tile, 2,197 at od = 3,
+// while most cells hold a run or two), else as kPairSparse.
+constexpr int kPairAuto = 0;
 constexpr int kPairSparse = 1;
 constexpr int kPairAll = 2;
+constexpr int kPairAllMinD = 6;
 __host__ __device__ __forceinline__ bool sparse_group(int runs, int all_runs, int cells,
-                                                      int mode) {
+                                                      int mode, int d) {
     if (mode == kPairAll) return true;
-    return runs < kSparseRuns && (long long)all_runs < (long long)kSparseFactor * cells;
+    const bool sparse_grid = (long long)all_runs < (long long)kSparseFactor * cells;
+    if (mode == kPairAuto && d >= kPairAllMinD) return sparse_grid;
+    return runs < kSparseRuns && sparse_grid;
 }
 
 // Grid geometry and thresholds, computed on the host (EGG_SynC.cu:1249-1260).
@@ -86,7 +93,7 @@ struct Geo {
     int qblocks;       // workgroups of the tile-queue kernels (0 = automatic)
     int qthreads;      // threads per workgroup of those kernels (64..256, multiple of 64)
     int gw;            // update-tile groups: cells per block of the last coordinate (<= 1: one cell)
-    int pair_mode;     // which groups the pair update takes (kPairSparse / kPairAll)
+    int pair_mode;     // which groups the pair update takes (kPairAuto / kPairSparse / kPairAll)
 };
 
 // Sub-cells: every cell is split in two along each dimension at its centre

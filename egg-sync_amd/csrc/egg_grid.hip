@@ -406,13 +406,13 @@ __global__ __launch_bounds__(kBlock) void k_group_counts(const int *__restrict__
                                                          int *__restrict__ ntile,
                                                          int *__restrict__ untile,
                                                          int *__restrict__ sparse_runs,
-                                                         int mode, DevState *st_loop) {
+                                                         int mode, int d, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int G = st->G;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
         const int runs = cell_run[grp_cell[k + 1]] - cell_run[grp_cell[k]];
         const int t = (runs + kTile - 1) / kTile;
-        const bool sp = sparse_group(runs, st->runs, st->C, mode);
+        const bool sp = sparse_group(runs, st->runs, st->C, mode, d);
         ntile[k] = t;
         untile[k] = sp ? 0 : t;
         sparse_runs[k] = sp ? runs : 0;
@@ -568,20 +568,23 @@ __global__ void k_store_total(const int *__restrict__ off, const int *__restrict
     *out = m > 0 ? off[m - 1] + cnt[m - 1] : 0;
 }
 
-// Runs of the sparse groups (sparse_group), listed for k_update_pairs at the
-// offsets of the scan of their counts.
-__global__ __launch_bounds__(kBlock) void k_sparse_list(const int *__restrict__ cell_run,
-                                                        const int *__restrict__ grp_cell,
-                                                        const int *__restrict__ off,
-                                                        const DevState *st,
-                                                        int *__restrict__ list, int mode,
-                                                        DevState *st_loop) {
+// The runs of the groups sparse_group selects, listed for k_update_pairs at
+// the offsets of the scan of their counts: one thread per run (its group is
+// gpos[c] + gflag[c] - 1 of its cell c).
+__global__ __launch_bounds__(kBlock) void k_sparse_list(
+    const int *__restrict__ run_start, const int *__restrict__ point_cell,
+    const int *__restrict__ gflag, const int *__restrict__ gpos,
+    const int *__restrict__ cell_run, const int *__restrict__ grp_cell,
+    const int *__restrict__ off, const DevState *st, int *__restrict__ list, int mode, int d,
+    DevState *st_loop) {
     if (st_loop && st_loop->done) return;
-    const int G = st->G;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < G; k += gridDim.x * blockDim.x) {
-        const int r0 = cell_run[grp_cell[k]], runs = cell_run[grp_cell[k + 1]] - r0;
-        if (sparse_group(runs, st->runs, st->C, mode))
-            for (int j = 0; j < runs; j++) list[off[k] + j] = r0 + j;
+    const int runs = st->runs;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < runs; r += gridDim.x * blockDim.x) {
+        const int c = point_cell[run_start[r]];
+        const int k = gpos[c] + gflag[c] - 1;
+        const int r0 = cell_run[grp_cell[k]];
+        if (sparse_group(cell_run[grp_cell[k + 1]] - r0, runs, st->C, mode, d))
+            list[off[k] + r - r0] = r;
     }
 }
 
@@ -639,7 +642,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     scan_exclusive(b.gflag, b.gpos, n, &st->C, b.scan_tmp, &st->G, st_loop, s);
     k_group_cells<<<gb, kBlock, 0, s>>>(b.gflag, b.gpos, st, b.grp_cell, st_loop);
     k_group_counts<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.ntile, b.untile,
-                                         b.sparse_runs, g.pair_mode, st_loop);
+                                         b.sparse_runs, g.pair_mode, g.d, st_loop);
     {
         ScanSet ss{};
         ss.in[0] = b.ntile;
@@ -655,8 +658,9 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     }
     // the runs of the pair update's groups (every run, in order, in mode kPairAll)
     if (g.pair_mode != kPairAll)
-        k_sparse_list<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, b.sparse_off, st,
-                                            b.sparse_runs, g.pair_mode, st_loop);
+        k_sparse_list<<<gb, kBlock, 0, s>>>(b.run_start, b.point_cell, b.gflag, b.gpos,
+                                            b.cell_run, b.grp_cell, b.sparse_off, st,
+                                            b.sparse_runs, g.pair_mode, g.d, st_loop);
     if (g.sub_bits) {
         // sub-cell runs and totals (anchors on the half-side grid)
         k_sub_runs<<<gb, kBlock, 0, s>>>(b.sub_start, b.rpos, st, b.sub_run, st_loop);

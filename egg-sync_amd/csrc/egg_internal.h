@@ -85,13 +85,18 @@ struct GridBufs {
     int *untile, *utile_off;// [n] the same for the groups of k_update_tile (not sparse)
     float *rres;         // [d][n] new position of each run
     int *sparse_runs;    // [n]   runs of sparse cell groups
-    // pair update (k_update_pairs): per run its cell and a flag (bit 1: a
-    // coordinate clamped into cell w-1); bounding boxes of blocks of kBlkRuns
-    // consecutive runs and of superblocks of kSupBlks blocks, and whether
-    // they hold a clamped run
+    // pair update (k_update_pairs): per run its cell, flags (bit 0: first
+    // run of its cell, bit 1: a coordinate clamped into cell w-1), position,
+    // cell coordinates and length; bounding boxes of blocks of kBlkRuns
+    // consecutive runs and of superblocks of kSupBlks blocks, whether they
+    // hold a clamped run, and the cell of a block whose runs share one (-1
+    // otherwise)
     int *rcell, *rflag;  // [n]
+    float *Xr;           // [d][n]
+    int *gr;             // [d][n]
+    int *rlen;           // [n]
     float *bmin, *bmax;  // [d][n / kBlkRuns + 1]
-    int *bclamp;         // [n / kBlkRuns + 1]
+    int *bclamp, *bcell; // [n / kBlkRuns + 1]
     float *sbmin, *sbmax;// [d][n / (kBlkRuns * kSupBlks) + 1]
     int *sbclamp;        // [n / (kBlkRuns * kSupBlks) + 1]
     int *sparse_off;     // [n]   their offsets per group (exclusive scan)

@@ -191,11 +191,15 @@ Layout carve(void *base, const Geo &g) {
     b.sparse_runs = c.take<int>(n);
     b.rcell = c.take<int>(n);
     b.rflag = c.take<int>(n);
+    b.Xr = c.take<float>(n * d);
+    b.gr = c.take<int>(n * d);
+    b.rlen = c.take<int>(n);
     {
         const size_t nb = n / kBlkRuns + 1, ns = n / ((size_t)kBlkRuns * kSupBlks) + 1;
         b.bmin = c.take<float>(nb * d);
         b.bmax = c.take<float>(nb * d);
         b.bclamp = c.take<int>(nb);
+        b.bcell = c.take<int>(nb);
         b.sbmin = c.take<float>(ns * d);
         b.sbmax = c.take<float>(ns * d);
         b.sbclamp = c.take<int>(ns);
@@ -382,7 +386,9 @@ void apply_tuning(const egg_sync_options &opt, Geo &g) {
     gw = std::min(gw, kAnchorBlock);
     while (gw & (gw - 1)) gw &= gw - 1;  // a power of two: groups never straddle an anchor block
     g.gw = gw;
-    g.pair_mode = opt.pair_groups == kPairAll ? kPairAll : kPairSparse;
+    g.pair_mode = opt.pair_groups == kPairAll      ? kPairAll
+                  : opt.pair_groups == kPairSparse ? kPairSparse
+                                                   : kPairAuto;
 }
 
 template <typename T>

@@ -959,17 +959,20 @@ __global__ __launch_bounds__(kBlock) void k_update_finish(
 // added in the oracle's order (runs weighted by their length); N and half are
 // exactly the reference's.
 
-// Per run: its cell and flags (bit 0: the first run of its cell, bit 1: a
-// coordinate clamped into cell w-1); per block of kBlkRuns runs: the bounding
-// box of its positions and whether it holds a clamped run (one wavefront per
-// block).
+// Per run: its cell, flags (bit 0: the first run of its cell, bit 1: a
+// coordinate clamped into cell w-1), position, cell coordinates and length,
+// indexed by run so that the pair update stages a block with one coalesced
+// load per field; per block of kBlkRuns runs: the bounding box of its
+// positions, whether it holds a clamped run, and its cell if all its runs
+// share one (-1 otherwise).  One wavefront per block.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_run_boxes(
     const float *__restrict__ Xs, const int *__restrict__ run_start,
-    const int *__restrict__ point_cell, const int *__restrict__ cell_run, Geo g, DevState *st,
-    int *__restrict__ rcell,
-    int *__restrict__ rflag, float *__restrict__ bmin, float *__restrict__ bmax,
-    int *__restrict__ bclamp) {
+    const int *__restrict__ point_cell, const int *__restrict__ cell_run,
+    const int *__restrict__ cell_g, Geo g, DevState *st, int *__restrict__ rcell,
+    int *__restrict__ rflag, float *__restrict__ Xr, int *__restrict__ gr,
+    int *__restrict__ rlen, float *__restrict__ bmin, float *__restrict__ bmax,
+    int *__restrict__ bclamp, int *__restrict__ bcell) {
     if (st->done || st->sparse_count == 0) return;
     const int n = g.n, runs = st->runs;
     const int nblk = (runs + kBlkRuns - 1) / kBlkRuns, nb = n / kBlkRuns + 1;
@@ -978,20 +981,23 @@ __global__ __launch_bounds__(kBlock) void k_run_boxes(
         const int r = b * kBlkRuns + __lane_id();
         const bool on = r < runs;
         float lo[D], hi[D];
-        int f = 0;
+        int f = 0, c = -1;
         if (on) {
             const int i = run_start[r];
-            const int c = point_cell[i];
+            c = point_cell[i];
             bool clamped = false;
 #pragma unroll
             for (int l = 0; l < D; l++) {
                 const float x = Xs[(long long)l * n + i];
                 lo[l] = hi[l] = x;
                 clamped |= x / g.cs >= wf;  // cell_coord's clamp of w to w-1
+                Xr[(long long)l * n + r] = x;
+                gr[(long long)l * n + r] = cell_g[(long long)l * n + c];
             }
             f = (cell_run[c] == r ? 1 : 0) | (clamped ? 2 : 0);
             rcell[r] = c;
             rflag[r] = f;
+            rlen[r] = run_start[r + 1] - i;
         } else {
 #pragma unroll
             for (int l = 0; l < D; l++) {
@@ -1000,6 +1006,8 @@ __global__ __launch_bounds__(kBlock) void k_run_boxes(
             }
         }
         const bool any_clamp = __any((f & 2) != 0);
+        const int c_first = __builtin_amdgcn_readfirstlane(c);
+        const bool one_cell = !__any(on && c != c_first);
 #pragma unroll
         for (int l = 0; l < D; l++) {
             for (int o = kWave / 2; o > 0; o >>= 1) {
@@ -1011,7 +1019,10 @@ __global__ __launch_bounds__(kBlock) void k_run_boxes(
                 bmax[(long long)l * nb + b] = hi[l];
             }
         }
-        if (__lane_id() == 0) bclamp[b] = any_clamp ? 1 : 0;
+        if (__lane_id() == 0) {
+            bclamp[b] = any_clamp ? 1 : 0;
+            bcell[b] = one_cell ? c_first : -1;
+        }
     }
 }
 
@@ -1101,10 +1112,12 @@ __global__ __launch_bounds__(kBlock) void k_update_pairs(
     const float *__restrict__ Xs, const int *__restrict__ cell_start,
     const int *__restrict__ cell_g, const float *__restrict__ Ts, const float *__restrict__ Tc,
     const int *__restrict__ run_start, const int *__restrict__ point_cell,
-    const int *__restrict__ list,
+    const int *__restrict__ cell_run, const int *__restrict__ list,
     const int *__restrict__ rcell, const int *__restrict__ rflag,
+    const float *__restrict__ Xr, const int *__restrict__ gr, const int *__restrict__ rlen,
     const float *__restrict__ bmin, const float *__restrict__ bmax,
-    const int *__restrict__ bclamp, const float *__restrict__ sbmin,
+    const int *__restrict__ bclamp, const int *__restrict__ bcell,
+    const float *__restrict__ sbmin,
     const float *__restrict__ sbmax, const int *__restrict__ sbclamp, float thr_filter,
     float thr_clamp, float *__restrict__ rres, int *__restrict__ rdbg, Geo g, DevState *st) {
     if (st->done) return;
@@ -1145,6 +1158,9 @@ __global__ __launch_bounds__(kBlock) void k_update_pairs(
             }
         }
         int N = 0, half = 0;
+        // the lane's decision for the last cell it classified: a cell's runs
+        // are consecutive in block order, so one box test serves all of them
+        int last_c = -1, last_cls = 0;  // 0 not counted, 1 full, 2 partial
         for (int sb0 = 0; sb0 < nsup; sb0 += kWave) {
             const int sbl = sb0 + lane;
             const bool sp = sbl < nsup &&
@@ -1158,17 +1174,58 @@ __global__ __launch_bounds__(kBlock) void k_update_pairs(
                                             bclamp[bl] ? box_thr_clamp : box_thr);
                 for (unsigned long long mb = __ballot(bp); mb; mb &= mb - 1) {
                     const int b = sb * kSupBlks + __ffsll((long long)mb) - 1;
-                    const int rr = b * kBlkRuns + lane;
+                    const int r0 = b * kBlkRuns;
+                    // a block whose runs share one cell c (a dense cell): one
+                    // box test of c per lane decides for all its runs, unless
+                    // c is partial for the lane or a border lane needs its
+                    // eps/2 count; if no lane needs the runs one by one, the
+                    // block is not streamed, and lanes for which c is full add
+                    // its totals in the block holding its first run
+                    const int bc = bcell[b];
+                    if (bc >= 0 && !bclamp[b]) {
+                        bool need = false, add_full = false;
+                        if (active) {
+                            int gc[D];
+                            bool win = true;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                gc[l] = gr[(long long)l * n + r0];
+                                if (l < od && (gc[l] - g0[l] > R || g0[l] - gc[l] > R)) win = false;
+                            }
+                            if (win) {
+                                float sm, lg;
+                                box_dist<D>(gc, x, g.cs, sm, lg);
+                                const bool full = lg <= thr;
+                                need = (!full && sm <= thr) || (full && border && bc != c0);
+                                add_full = full;
+                            }
+                        }
+                        if (!__any(need)) {
+                            const int fr = cell_run[bc];
+                            if (add_full && fr >= r0 && fr < r0 + kBlkRuns) {
+                                const int cnt = cell_start[bc + 1] - cell_start[bc];
+#pragma unroll
+                                for (int l = 0; l < D; l++) {
+                                    const float a = Ts[(long long)l * n + bc] * cosf(x[l]);
+                                    const float bb = Tc[(long long)l * n + bc] * sinf(x[l]);
+                                    const float tt = a - bb;
+                                    sum[l] += tt;
+                                }
+                                N += cnt;
+                                if (border && bc == c0) half += cnt;
+                            }
+                            continue;
+                        }
+                    }
+                    const int rr = r0 + lane;
                     if (rr < runs) {
-                        const int j = run_start[rr];
-                        const int c = rcell[rr];
-                        S.len[lane] = run_start[rr + 1] - j;
+                        S.len[lane] = rlen[rr];
                         S.flag[lane] = rflag[rr];
-                        S.cell[lane] = c;
+                        S.cell[lane] = rcell[rr];
 #pragma unroll
                         for (int l = 0; l < D; l++) {
-                            const int gl = cell_g[(long long)l * n + c];
-                            const float xq = Xs[(long long)l * n + j];
+                            const int gl = gr[(long long)l * n + rr];
+                            const float xq = Xr[(long long)l * n + rr];
                             S.xr[lane][l] = xq;
                             S.x[l][lane] = xq;
                             S.g[l][lane] = gl;
@@ -1200,19 +1257,18 @@ __global__ __launch_bounds__(kBlock) void k_update_pairs(
                         if (acc) {
                             const int q = __ffsll((long long)acc) - 1;
                             acc &= acc - 1;
-                            float s = 0.f;
-                            bool win = true;
+                            const int c = S.cell[q];
+                            if (c != last_c) {
+                                // the box test of the source's cell, EGG_SynC.cu:612-634,
+                                // for a cell of the reference's neighbour list
+                                bool win = true;
 #pragma unroll
-                            for (int l = 0; l < D; l++) {
-                                const float df = x[l] - S.x[l][q];
-                                s += df * df;
-                                if (l < od) {
-                                    const int gq = S.g[l][q];
-                                    if (gq - g0[l] > R || g0[l] - gq > R) win = false;
+                                for (int l = 0; l < D; l++) {
+                                    if (l < od) {
+                                        const int gq = S.g[l][q];
+                                        if (gq - g0[l] > R || g0[l] - gq > R) win = false;
+                                    }
                                 }
-                            }
-                            if (win) {
-                                // the box test of the source's cell, EGG_SynC.cu:612-634
                                 float sm = 0.f, lg = 0.f;
 #pragma unroll
                                 for (int l = 0; l < D; l++) {
@@ -1228,23 +1284,40 @@ __global__ __launch_bounds__(kBlock) void k_update_pairs(
                                         lg += lo;
                                     }
                                 }
-                                const int c = S.cell[q];
-                                if (lg <= thr) {
-                                    // full: totals and count once, at the cell's first run
-                                    if (S.flag[q] & 1) {
-                                        const int cnt = cell_start[c + 1] - cell_start[c];
+                                last_c = c;
+                                last_cls = !win ? 0 : (lg <= thr ? 1 : (sm <= thr ? 2 : 0));
+                            }
+                            if (last_cls == 1) {
+                                // full: totals and count once, at the cell's first run
+                                if (S.flag[q] & 1) {
+                                    const int cnt = cell_start[c + 1] - cell_start[c];
 #pragma unroll
-                                        for (int l = 0; l < D; l++) {
-                                            const float a = Ts[(long long)l * n + c] * cosf(x[l]);
-                                            const float bb = Tc[(long long)l * n + c] * sinf(x[l]);
-                                            const float tt = a - bb;
-                                            sum[l] += tt;
-                                        }
-                                        N += cnt;
-                                        if (border && c == c0) half += cnt;
+                                    for (int l = 0; l < D; l++) {
+                                        const float a = Ts[(long long)l * n + c] * cosf(x[l]);
+                                        const float bb = Tc[(long long)l * n + c] * sinf(x[l]);
+                                        const float tt = a - bb;
+                                        sum[l] += tt;
                                     }
-                                    if (border && c != c0 && s <= thr_half) half += S.len[q];
-                                } else if (sm <= thr && s <= thr) {
+                                    N += cnt;
+                                    if (border && c == c0) half += cnt;
+                                }
+                                if (border && c != c0) {
+                                    float s = 0.f;
+#pragma unroll
+                                    for (int l = 0; l < D; l++) {
+                                        const float df = x[l] - S.x[l][q];
+                                        s += df * df;
+                                    }
+                                    if (s <= thr_half) half += S.len[q];
+                                }
+                            } else if (last_cls == 2) {
+                                float s = 0.f;
+#pragma unroll
+                                for (int l = 0; l < D; l++) {
+                                    const float df = x[l] - S.x[l][q];
+                                    s += df * df;
+                                }
+                                if (s <= thr) {
                                     const int lq = S.len[q];
                                     const float w = (float)lq;  // exact: lq < 2^24
 #pragma unroll
@@ -1718,9 +1791,10 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
     // every filtered pair is closer than 1 in each coordinate
     const bool pair_small = thr_clamp < 1.0f;
 #define PAIR_ARGS                                                                               \
-    b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.run_start, b.point_cell,                        \
-        g.pair_mode == kPairAll ? nullptr : b.sparse_runs, b.rcell, b.rflag, b.bmin, b.bmax,    \
-        b.bclamp, b.sbmin, b.sbmax, b.sbclamp, thr_filter, thr_clamp, b.rres, rdbg, g, st
+    b.Xs, b.cell_start, b.cell_g, b.Ts, b.Tc, b.run_start, b.point_cell, b.cell_run,            \
+        g.pair_mode == kPairAll ? nullptr : b.sparse_runs, b.rcell, b.rflag, b.Xr, b.gr,        \
+        b.rlen, b.bmin, b.bmax, b.bclamp, b.bcell, b.sbmin, b.sbmax, b.sbclamp, thr_filter,     \
+        thr_clamp, b.rres, rdbg, g, st
     int *rdbg = (d.N || d.half || d.border || d.synced) ? b.rdbg : nullptr;
 #define UPD_CASE(DD)                                                                          \
     case DD:                                                                                  \
@@ -1740,9 +1814,9 @@ void launch_update_egg(const Geo &g, GridBufs &b, DevState *st, const DebugOut *
                                                   b.utile_off, b.dtiles, b.rdbg, b.hacc,      \
                                                   b.rres, rdbg, g, st);                       \
         }                                                                                     \
-        k_run_boxes<DD><<<pb, kBlock, 0, s>>>(b.Xs, b.run_start, b.point_cell, b.cell_run, g,   \
-                                              st, b.rcell,                                    \
-                                              b.rflag, b.bmin, b.bmax, b.bclamp);             \
+        k_run_boxes<DD><<<pb, kBlock, 0, s>>>(b.Xs, b.run_start, b.point_cell, b.cell_run,     \
+                                              b.cell_g, g, st, b.rcell, b.rflag, b.Xr, b.gr,  \
+                                              b.rlen, b.bmin, b.bmax, b.bclamp, b.bcell);     \
         k_super_boxes<DD><<<sb, kBlock, 0, s>>>(b.bmin, b.bmax, b.bclamp, g, st, b.sbmin,     \
                                                 b.sbmax, b.sbclamp);                          \
         if (pair_small)                                                                       \

@@ -79,10 +79,12 @@ typedef struct egg_sync_options {
                                 share a wavefront (0 = default).  It changes the
                                 order of FP32 additions only                      */
     int32_t pair_groups;     /* cells updated point pair by point pair instead of
-                                by the cell walk: 0 = automatic, 1 = groups of
-                                fewer than 8 distinct positions in sparse grids,
-                                2 = every group.  It changes the order of FP32
-                                additions only                                    */
+                                by the cell walk: 1 = groups of fewer than 8
+                                distinct positions in sparse grids (fewer than 4
+                                distinct positions per cell), 2 = every group,
+                                0 = automatic: every group of a sparse grid from
+                                d = 6 on, else as 1.  It changes the order of
+                                FP32 additions only                               */
 } egg_sync_options;
 
 typedef struct egg_sync_stats {
