@@ -9,6 +9,10 @@ with the CPU oracle without re-running it:
                       64-instance generator, n=165,583, d=2, eps=0.1; ~11,600
                       iterations, ~3 min): labels, iteration count, sampled final
                       positions.
+  c1_fp64_oracle.npz  config 1 solved by the FP64 oracle: iteration count, labels, the
+                      final positions at c1_full_oracle.npz's sample indices, and the
+                      order parameter r_c of the final positions computed entirely in
+                      FP64 (what the benchmark reports the GPU's r_c and positions against).
   c4_full_oracle.npz  config 4 (n=4,000,000, d=3, eps=0.01), run pass by pass: labels,
                       iteration count, sampled final positions, column sums, and per
                       pass the oracle's flags and CPU seconds.
@@ -18,7 +22,7 @@ with the CPU oracle without re-running it:
                       the FP64 final positions computed entirely in FP64
                       (brute force over distinct positions with multiplicities).
 
-    python tests/golden/make_oracle_fixtures.py [c1] [c2] [c3_tail] [c4]
+    python tests/golden/make_oracle_fixtures.py [c1] [c1_fp64] [c2] [c3_tail] [c4]
 """
 import os
 import sys
@@ -74,6 +78,18 @@ def c2():
     print("c2 clusters", int(r["labels"].max()) + 1, "rc64", rc64, flush=True)
 
 
+def c1_fp64():
+    X, eps = data.config1()
+    r = O.egg_solve(X, eps, fp64=True)
+    idx = np.load(os.path.join(HERE, "c1_full_oracle.npz"))["sample_idx"]
+    rc64 = rc_fp64(r["X"], eps)
+    np.savez_compressed(os.path.join(HERE, "c1_fp64_oracle.npz"), labels=r["labels"],
+                        iterations=np.int32(r["iterations"]), sample_idx=idx,
+                        sample_X=r["X"][idx], rc64=np.float64(rc64))
+    print("c1 fp64 iterations", r["iterations"], "clusters", int(r["labels"].max()) + 1,
+          "rc64", rc64, flush=True)
+
+
 def c4():
     import time
     X, eps = data.config4()
@@ -115,4 +131,4 @@ if __name__ == "__main__":
     O.build()
     todo = sys.argv[1:] or ["c1", "c2", "c3_tail"]
     for name in todo:
-        {"c1": c1, "c2": c2, "c3_tail": c3_tail, "c4": c4}[name]()
+        {"c1": c1, "c1_fp64": c1_fp64, "c2": c2, "c3_tail": c3_tail, "c4": c4}[name]()
