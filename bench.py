@@ -373,6 +373,25 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     if full_size:
+        # the order parameter r_c of the final state (computed in FP64 on the
+        # GPU, SynC.cpp:42-56) and the final positions, against a run of the
+        # oracle entirely in FP64 (committed fixture, not run here)
+        from egg_sync import solver
+        from egg_sync.metrics import same_partition
+        a = solver.solve(D, eps, compute_rc=True)
+        g = np.load(os.path.join(ROOT, "tests", "golden", "c1_fp64_oracle.npz"))
+        idx = g["sample_idx"]
+        line["fp64_check"] = {
+            "rc": float(a["stats"]["rc"]), "rc_fp64_oracle": float(g["rc64"]),
+            "rc_abs_err": abs(float(a["stats"]["rc"]) - float(g["rc64"])),
+            "max_abs_pos_err": float(np.abs(a["X"][idx].astype(np.float64)
+                                            - g["sample_X"]).max()),
+            "positions_compared": int(len(idx)),
+            "iterations": int(a["stats"]["iterations"]),
+            "iterations_fp64_oracle": int(g["iterations"]),
+            "same_partition_as_fp64_oracle": bool(same_partition(a["labels"], g["labels"])),
+            "source": "tests/golden/c1_fp64_oracle.npz (tests/golden/make_oracle_fixtures.py "
+                      "c1_fp64: the oracle's solve with every operation in FP64)"}
         # the pass is bound by VALU instruction issue, not by HBM bytes: wave
         # instructions per pass (rocprofv3 SQ_INSTS_VALU over the same solve)
         # over the pass time measured here, against 256 CUs x 2.4 GHz x one

@@ -265,7 +265,7 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
     }
 #undef KEYS_CASE
     int where = radix_sort(b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, g.kw, g.sort_bits, b.hist,
-                           b.scan_tmp, st_loop, s);
+                           b.scan_tmp, st_loop, s, true);
     const uint64_t *skeys = where == 1 ? b.keys_b : b.keys_a;
     const int *sidx = where == 1 ? b.vals_b : (where == 0 ? b.vals_a : nullptr);
 #define GATHER_CASE(DD)                                                                  \
@@ -287,7 +287,7 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
         ss.out[1] = b.rpos;
         ss.in[2] = b.shead;
         ss.out[2] = b.spos;
-        scan_exclusive_multi(ss, g.sub_bits ? 3 : 2, n, nullptr, b.scan_tmp, st_loop, s);
+        scan_exclusive_multi(ss, g.sub_bits ? 3 : 2, n, nullptr, b.scan_tmp, st_loop, s, true);
     }
     launch_fill(b.ob, g.O, 0, st_loop, s);
     launch_fill(b.oe, g.O, 0, st_loop, s);
@@ -611,7 +611,7 @@ static void group_summaries(const Geo &g, GridBufs &b, const int *starts, const 
     const int n = g.n;
     const int gb = grid_for(n);
     k_wi_count<<<gb, kBlock, 0, s>>>(starts, count, nchunk, st_loop);
-    scan_exclusive(nchunk, wi_off, n, count, b.scan_tmp, nullptr, st_loop, s);
+    scan_exclusive(nchunk, wi_off, n, count, b.scan_tmp, nullptr, st_loop, s, true);
     k_store_total<<<1, 1, 0, s>>>(wi_off, nchunk, count, nwork, st_loop);
     // work items <= groups + n / kSumChunk <= n + n / kSumChunk
     const long long max_wi = (long long)n + n / kSumChunk + 1;
@@ -639,7 +639,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     k_runs<<<gb, kBlock, 0, s>>>(b.rhead, b.rpos, b.point_cell, b.cell_start, n, b.run_start,
                                  b.cell_run, st, st_loop);
     k_group_heads<<<gb, kBlock, 0, s>>>(b.cell_g, b.cell_outer, g, st, b.gflag, st_loop);
-    scan_exclusive(b.gflag, b.gpos, n, &st->C, b.scan_tmp, &st->G, st_loop, s);
+    scan_exclusive(b.gflag, b.gpos, n, &st->C, b.scan_tmp, &st->G, st_loop, s, true);
     k_group_cells<<<gb, kBlock, 0, s>>>(b.gflag, b.gpos, st, b.grp_cell, st_loop);
     k_group_counts<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.ntile, b.untile,
                                          b.sparse_runs, g.pair_mode, g.d, st_loop);
@@ -654,7 +654,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
         ss.in[2] = b.sparse_runs;
         ss.out[2] = b.sparse_off;
         ss.total[2] = &st->sparse_count;
-        scan_exclusive_multi(ss, 3, n, &st->G, b.scan_tmp, st_loop, s);
+        scan_exclusive_multi(ss, 3, n, &st->G, b.scan_tmp, st_loop, s, true);
     }
     // the runs of the pair update's groups (every run, in order, in mode kPairAll)
     if (g.pair_mode != kPairAll)

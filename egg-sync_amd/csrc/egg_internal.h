@@ -8,9 +8,17 @@ namespace egg {
 // Exclusive scan of n int32 values (n_dev, if given, overrides n at run time;
 // it must not exceed n).  `tmp` needs scan_tmp_ints(n) ints.  The sum of all
 // values is written to *total if total != nullptr.  In-place allowed.
+// With `lookback`, arrays of more than one tile are scanned in one launch
+// (decoupled look-back); tmp then holds the scans' shared state and must be
+// zero before the first such scan (scan_tmp_reset) -- each scan leaves it
+// ready for the next on the same stream.
 size_t scan_tmp_ints(int n);
 void scan_exclusive(const int *in, int *out, int n, const int *n_dev, int *tmp, int *total,
-                    const DevState *st, hipStream_t s);
+                    const DevState *st, hipStream_t s, bool lookback = false);
+// Zeroes the look-back state in tmp (scan_tmp_ints(n) ints); not for graph capture.
+int scan_tmp_reset(int *tmp, int n, hipStream_t s);
+// Nonzero if a look-back scan using tmp gave up waiting (an internal error).
+int scan_tmp_error(const int *tmp, hipStream_t s, int *err);
 
 // Exclusive scans of nch (<= kMaxScanCh) int32 arrays of the same length in
 // one pass: channel c scans in[c] into out[c] (in place allowed) and writes
@@ -22,7 +30,7 @@ struct ScanSet {
     int *total[kMaxScanCh];
 };
 void scan_exclusive_multi(const ScanSet &ss, int nch, int n, const int *n_dev, int *tmp,
-                          const DevState *st, hipStream_t s);
+                          const DevState *st, hipStream_t s, bool lookback = false);
 
 // Stable LSD radix sort of (key, position) pairs by key bits [0, key_bits),
 // 8 bits per pass.  keys_a/keys_b: [kw][n] u64; vals_a/vals_b: [n] int32.
@@ -30,7 +38,8 @@ void scan_exclusive_multi(const ScanSet &ss, int nch, int n, const int *n_dev, i
 // the sorted result is in the *_a buffers, 1 if in *_b.
 size_t sort_tmp_ints(int n);
 int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int n, int kw,
-               int key_bits, int *hist, int *scan_tmp, const DevState *st, hipStream_t s);
+               int key_bits, int *hist, int *scan_tmp, const DevState *st, hipStream_t s,
+               bool lookback = false);
 
 // A heavy streamed range of an update tile: tile t, candidate cell c, the
 // range's runs (runs [r0, r0 + total) if smask is 0, else the runs of

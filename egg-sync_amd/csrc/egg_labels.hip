@@ -36,7 +36,7 @@ void launch_labels_egg(const Geo &g, GridBufs &b, int *labels, int *multi, int *
                        DevState *st, hipStream_t s) {
     const int gb = grid_for(g.n);
     k_label_flags<<<gb, kBlock, 0, s>>>(b.cell_start, st, multi);
-    scan_exclusive(multi, lab_off, g.n, &st->C, b.scan_tmp, &st->n_clusters, nullptr, s);
+    scan_exclusive(multi, lab_off, g.n, &st->C, b.scan_tmp, &st->n_clusters, nullptr, s, true);
     k_labels_egg<<<gb, kBlock, 0, s>>>(b.point_cell, multi, lab_off, b.perm_s, g.n, labels);
 }
 

@@ -7,6 +7,9 @@
 // each wavefront ranks its 64 keys by digit with 8 ballots (lanes holding the
 // same digit form a 64-bit match mask).  Stability makes the grid build
 // deterministic: points that share a cell keep their previous order.
+#include <algorithm>
+
+#include "egg_host.h"
 #include "egg_internal.h"
 
 namespace egg {
@@ -138,18 +141,177 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(ScanSet ss, int n, const i
     }
 }
 
+// ------------------------------------------------- single-pass scan --
+//
+// Decoupled look-back (Merrill and Garland, "Single-pass Parallel Prefix Scan
+// with Decoupled Look-back", 2016).  Each workgroup takes a tile from a
+// ticket counter (so a tile's predecessors belong to workgroups that started
+// earlier), scans it locally and publishes its sum; wavefront 0 then walks
+// back over its predecessors' published words, 64 at a time, adding sums
+// until it meets an inclusive prefix, and publishes its own inclusive prefix.
+// A predecessor publishes its sum before it waits on anything, so every wait
+// ends; the wait is still bounded (kLbSpins polls, then an error flag and a
+// wrong prefix, reported to the host by scan_tmp_error).
+//
+// State in tmp: [0] ticket, [1] workgroups finished, [2] scan sequence
+// number, [3] error flag; from int 4 on, one 64-bit word per channel and
+// tile: flag (bits 63-62: 1 sum, 2 inclusive prefix), the sequence number of
+// the scan that wrote it (bits 61-32, so that words of earlier scans read as
+// unpublished) and the value (bits 31-0).  The last workgroup to finish
+// resets the counters and advances the sequence number.
+constexpr int kLbSpins = 1 << 20;
+constexpr unsigned long long kLbSum = 1ull << 62, kLbIncl = 2ull << 62;
+
+__device__ __forceinline__ unsigned long long lb_word(unsigned long long flag, unsigned seq,
+                                                      int v) {
+    return flag | ((unsigned long long)(seq & 0x3fffffffu) << 32) | (unsigned)v;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Exclusive prefix of tile `tile` (> 0) from its predecessors' words; one
+// wavefront.
+__device__ int lb_prefix(const unsigned long long *status, int tile, unsigned seq, int *err) {
+    const int lane = __lane_id();
+    const unsigned tag = seq & 0x3fffffffu;
+    int pre = 0, spins = 0;
+    for (int j = tile - 1;;) {
+        const int idx = j - lane;
+        int flag = 2, val = 0;  // before tile 0: an inclusive prefix of 0
+        if (idx >= 0) {
+            const unsigned long long w =
+                __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            flag = (unsigned)((w >> 32) & 0x3fffffffu) == tag ? (int)(w >> 62) : 0;
+            val = (int)(unsigned)w;
+        }
+        const unsigned long long incl = __ballot(flag == 2);
+        const unsigned long long none = __ballot(flag == 0);
+        const int stop = incl ? __ffsll((long long)incl) - 1 : kWave - 1;
+        const unsigned long long need = stop == kWave - 1 ? ~0ull : ((2ull << stop) - 1ull);
+        if (none & need) {
+            if (++spins > kLbSpins) {
+                if (lane == 0) atomicOr(err, 1);
+                return pre;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        pre += wave_sum(lane <= stop ? val : 0);
+        if (incl) return pre;
+        j -= kWave;
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_scan_lookback(ScanSet ss, int n, const int *n_dev,
+                                                          int *lb, const DevState *st) {
+    if (st && st->done) return;
+    __shared__ int s_w[NC][kBlock / kWave];
+    __shared__ int s_tile, s_pre[NC];
+    unsigned long long *status = (unsigned long long *)(lb + 4);
+    const int nb = gridDim.x;
+    if (threadIdx.x == 0) s_tile = atomicAdd(&lb[0], 1);
+    __syncthreads();
+    const int tile = s_tile;
+    const unsigned seq =
+        __hip_atomic_load((const unsigned *)&lb[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (n_dev) n = min(n, *n_dev);
+    const long long tile0 = (long long)tile * kScanTile;
+    if (n <= 0 && tile == 0 && threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+            if (ss.total[c]) *ss.total[c] = 0;
+    }
+    if (tile0 < n) {
+        const long long base = tile0 + (long long)threadIdx.x * kScanItems;
+        int vals[NC][kScanItems], ex[NC], tot[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            int v = 0;
+#pragma unroll
+            for (int k = 0; k < kScanItems; k++) {
+                vals[c][k] = (base + k < n) ? ss.in[c][base + k] : 0;
+                v += vals[c][k];
+            }
+            ex[c] = block_excl_scan(v, s_w[c], tot[c]);
+        }
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                __hip_atomic_store(status + (long long)c * nb + tile,
+                                   lb_word(tile == 0 ? kLbIncl : kLbSum, seq, tot[c]),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (threadIdx.x < kWave) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const int pre = tile == 0 ? 0 : lb_prefix(status + (long long)c * nb, tile, seq,
+                                                          &lb[3]);
+                if (threadIdx.x == 0) {
+                    s_pre[c] = pre;
+                    if (tile > 0)
+                        __hip_atomic_store(status + (long long)c * nb + tile,
+                                           lb_word(kLbIncl, seq, pre + tot[c]), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        __syncthreads();
+        const bool last = tile0 + kScanTile >= n;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            int e = ex[c] + s_pre[c];
+#pragma unroll
+            for (int k = 0; k < kScanItems; k++) {
+                if (base + k < n) ss.out[c][base + k] = e;
+                e += vals[c][k];
+            }
+            if (last && threadIdx.x == 0 && ss.total[c]) *ss.total[c] = s_pre[c] + tot[c];
+        }
+    }
+    if (threadIdx.x == 0) {
+        const int fin = atomicAdd(&lb[1], 1);
+        if (fin == nb - 1) {
+            __hip_atomic_store(&lb[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&lb[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store((unsigned *)&lb[2], seq + 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 size_t scan_tmp_ints(int n) {
-    return (size_t)kMaxScanCh * ((size_t)((n + kScanTile - 1) / kScanTile) + 1);
+    const size_t nb = (size_t)((n + kScanTile - 1) / kScanTile) + 1;
+    // block sums of the three-launch scan, or the look-back state
+    return std::max((size_t)kMaxScanCh * nb, 4 + 2 * (size_t)kMaxScanCh * nb);
+}
+
+int scan_tmp_reset(int *tmp, int n, hipStream_t s) {
+    return hip_err(hipMemsetAsync(tmp, 0, sizeof(int) * scan_tmp_ints(n), s));
+}
+
+int scan_tmp_error(const int *tmp, hipStream_t s, int *err) {
+    int rc = hip_err(hipMemcpyAsync(err, tmp + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (!rc) rc = hip_err(hipStreamSynchronize(s));
+    return rc;
 }
 
 template <int NC>
 static void scan_launch(const ScanSet &ss, int n, const int *n_dev, int *tmp, const DevState *st,
-                        hipStream_t s) {
+                        hipStream_t s, bool lookback) {
     if (n <= kScanTile) {
         k_scan_single<NC><<<1, kSingleBlock, 0, s>>>(ss, n, n_dev, st);
         return;
     }
     const int nb = (n + kScanTile - 1) / kScanTile;
+    if (lookback) {
+        k_scan_lookback<NC><<<nb, kBlock, 0, s>>>(ss, n, n_dev, tmp, st);
+        return;
+    }
     k_scan_reduce<NC><<<nb, kBlock, 0, s>>>(ss, n, n_dev, tmp, nb, st);
     ScanSet bs{};
     for (int c = 0; c < NC; c++) {
@@ -162,22 +324,22 @@ static void scan_launch(const ScanSet &ss, int n, const int *n_dev, int *tmp, co
 }
 
 void scan_exclusive_multi(const ScanSet &ss, int nch, int n, const int *n_dev, int *tmp,
-                          const DevState *st, hipStream_t s) {
+                          const DevState *st, hipStream_t s, bool lookback) {
     switch (nch) {
-        case 1: scan_launch<1>(ss, n, n_dev, tmp, st, s); break;
-        case 2: scan_launch<2>(ss, n, n_dev, tmp, st, s); break;
-        case 3: scan_launch<3>(ss, n, n_dev, tmp, st, s); break;
-        default: scan_launch<4>(ss, n, n_dev, tmp, st, s); break;
+        case 1: scan_launch<1>(ss, n, n_dev, tmp, st, s, lookback); break;
+        case 2: scan_launch<2>(ss, n, n_dev, tmp, st, s, lookback); break;
+        case 3: scan_launch<3>(ss, n, n_dev, tmp, st, s, lookback); break;
+        default: scan_launch<4>(ss, n, n_dev, tmp, st, s, lookback); break;
     }
 }
 
 void scan_exclusive(const int *in, int *out, int n, const int *n_dev, int *tmp, int *total,
-                    const DevState *st, hipStream_t s) {
+                    const DevState *st, hipStream_t s, bool lookback) {
     ScanSet ss{};
     ss.in[0] = in;
     ss.out[0] = out;
     ss.total[0] = total;
-    scan_launch<1>(ss, n, n_dev, tmp, st, s);
+    scan_launch<1>(ss, n, n_dev, tmp, st, s, lookback);
 }
 
 // ------------------------------------------------------------ radix sort --
@@ -271,7 +433,8 @@ size_t sort_tmp_ints(int n) {
 }
 
 int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int n, int kw,
-               int key_bits, int *hist, int *scan_tmp, const DevState *st, hipStream_t s) {
+               int key_bits, int *hist, int *scan_tmp, const DevState *st, hipStream_t s,
+               bool lookback) {
     int ntiles = (n + kSortTile - 1) / kSortTile;
     if (ntiles == 0) ntiles = 1;
     const int passes = (key_bits + 7) / 8;
@@ -281,7 +444,7 @@ int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int
     for (int p = 0; p < passes; p++) {
         const int shift = p * 8;
         k_rs_hist<<<ntiles, kBlock, 0, s>>>(kin, n, shift, hist, ntiles, st);
-        scan_exclusive(hist, hist, 256 * ntiles, nullptr, scan_tmp, nullptr, st, s);
+        scan_exclusive(hist, hist, 256 * ntiles, nullptr, scan_tmp, nullptr, st, s, lookback);
 #define RS_CASE(KW)                                                                          \
     case KW:                                                                                 \
         k_rs_scatter<KW><<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, hist,     \

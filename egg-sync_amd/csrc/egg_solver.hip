@@ -226,6 +226,12 @@ Layout carve(void *base, const Geo &g) {
     return L;
 }
 
+// The look-back scans of the grid build keep their state in scan_tmp; it
+// starts at zero for every solve (a workspace is reused across shapes).
+static int reset_scan_state(const Geo &g, Layout &L, hipStream_t s) {
+    return scan_tmp_reset(L.b.scan_tmp, (int)std::max((size_t)g.n, sort_tmp_ints(g.n)), s);
+}
+
 #define HIP_TRY(x)                       \
     do {                                 \
         int rc_ = hip_err((x));          \
@@ -543,6 +549,8 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
     DevState hs;
     init_state(hs, opt);
     HIP_TRY(hipMemcpyAsync(L.st, &hs, sizeof hs, hipMemcpyHostToDevice, s));
+    rc = reset_scan_state(g, L, s);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s));
     const auto t_alloc = clk::now();
 
@@ -639,6 +647,10 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
     if (fe) (void)hipEventRecord(fe[2], s);
     HIP_TRY(hipStreamSynchronize(s));
     const DevState fin = *h_state;
+    int scan_err = 0;
+    rc = scan_tmp_error(L.b.scan_tmp, s, &scan_err);
+    if (rc) return rc;
+    if (scan_err) return EGG_SYNC_E_INTERNAL;
     if (so) {
         so->iterations = loop_state.iter;
         so->status = loop_state.capped ? EGG_SYNC_CAPPED : EGG_SYNC_OK;
@@ -678,7 +690,7 @@ int setup_single(const float *d_D, int64_t n64, int d, float eps, const int32_t 
     DevState hs;
     init_state(hs, opt);
     HIP_TRY(hipMemcpyAsync(L.st, &hs, sizeof hs, hipMemcpyHostToDevice, s));
-    return 0;
+    return reset_scan_state(g, L, s);
 }
 
 // ------------------------------------------------------------ batched solve --
@@ -862,8 +874,11 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
     }
     HIP_TRY(hipMemcpyAsync(bw.states, bw.h_states, sizeof(DevState) * n_inst,
                            hipMemcpyHostToDevice, s));
-    for (int k : active)
+    for (int k : active) {
         launch_aos_to_soa(d_D + inst[k].data_offset, Ls[k].b.X, Ls[k].b.perm, G[k].n, G[k].d, s);
+        rc = reset_scan_state(G[k], Ls[k], s);
+        if (rc) return rc;
+    }
 
     // largest first onto the least-loaded stream
     std::vector<int> order = active;
@@ -1222,6 +1237,7 @@ const char *egg_sync_error_string(int code) {
         case EGG_SYNC_E_ALLOC: return "device allocation failed";
         case EGG_SYNC_E_HIP: return "HIP runtime error";
         case EGG_SYNC_E_NODEV: return "no GPU device";
+        case EGG_SYNC_E_INTERNAL: return "internal error: a bounded device-side wait expired";
         case EGG_SYNC_E_TOO_BIG:
             return "workload too large for the no-alloc scratch array (64 MiB of workspace)";
         default: return "unknown error";

@@ -13,7 +13,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libeggsync.so")
 
 OK, CAPPED = 0, 1
-E_ARG, E_RANGE, E_DIM, E_KEY, E_ALLOC, E_HIP, E_NODEV, E_TOO_BIG = -1, -2, -3, -4, -5, -6, -7, -8
+E_ARG, E_RANGE, E_DIM, E_KEY, E_ALLOC, E_HIP, E_NODEV, E_TOO_BIG, E_INTERNAL = (
+    -1, -2, -3, -4, -5, -6, -7, -8, -9)
 STOP_EGG, STOP_RC = 0, 1
 LABELS_CELLS, LABELS_SYNC = 0, 1
 MAX_D = 20

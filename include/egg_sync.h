@@ -49,6 +49,7 @@ extern "C" {
 #define EGG_SYNC_E_HIP (-6)         /* HIP runtime error                       */
 #define EGG_SYNC_E_NODEV (-7)       /* no GPU visible                          */
 #define EGG_SYNC_E_TOO_BIG (-8)     /* workspace beyond the no-alloc scratch   */
+#define EGG_SYNC_E_INTERNAL (-9)    /* a bounded device-side wait expired      */
 
 #define EGG_SYNC_STOP_EGG 0         /* local sync + MBR extra check (EGG_SynC.cu:706-709,1425-1489) */
 #define EGG_SYNC_STOP_RC 1          /* r_c >= lam (SynC.cpp:277)                                    */

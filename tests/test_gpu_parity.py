@@ -302,6 +302,23 @@ def test_full_size_config1_properties(S, O):
     assert np.abs(ga["X_next"] - gb["X_next"]).max() <= 1e-6
 
 
+def test_full_size_config1_against_fp64_oracle(S, O):
+    """n = 1,000,000, d = 3, eps = 0.02 against the oracle run entirely in FP64
+    (tests/golden/c1_fp64_oracle.npz): iterations, partition, positions, and
+    the FP64 order parameter r_c of the final state."""
+    from egg_sync import data
+    X, eps = data.config1()
+    g = golden("c1_fp64_oracle")
+    _say("config1 solve with r_c")
+    a = S.solve(X, eps, compute_rc=True)
+    assert a["stats"]["iterations"] == int(g["iterations"])
+    assert O.same_partition(a["labels"], g["labels"])
+    err = float(np.abs(a["X"][g["sample_idx"]].astype(np.float64) - g["sample_X"]).max())
+    drc = abs(a["stats"]["rc"] - float(g["rc64"]))
+    _say(f"config1: max |x - x_oracle64| {err:.3g}, |rc - rc_oracle64| {drc:.3g}")
+    assert err <= 1e-5 and drc <= 1e-6
+
+
 def test_config1_sample_solve_matches_oracle(S, O):
     from egg_sync import data
     X, eps = data.config1(n=100_000)

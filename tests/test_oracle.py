@@ -402,6 +402,21 @@ def test_rc_of_collapsed_clusters(O):
     assert abs(O.rc(X2, 0.05) - (1.0 + float(np.exp(f32(-dist)))) / 2.0) < 1e-7
 
 
+def test_config1_fp64_fixture_agrees_with_fp32_fixture():
+    """The oracle's n=1,000,000 solve run in FP64 and in FP32 (both committed,
+    written by tests/golden/make_oracle_fixtures.py): the same iteration count
+    and partition, positions within 1e-6 of each other, and an order parameter
+    r_c of 1 -- every point's eps-neighbours share its final position, so each
+    term exp(-|sin(x_q - x)|) is 1."""
+    from egg_sync.metrics import same_partition
+    a, b = golden("c1_fp64_oracle"), golden("c1_full_oracle")
+    assert int(a["iterations"]) == int(b["iterations"]) == 10
+    assert same_partition(a["labels"], b["labels"])
+    assert np.array_equal(a["sample_idx"], b["sample_idx"])
+    assert np.abs(a["sample_X"] - b["sample_X"].astype(np.float64)).max() <= 1e-6
+    assert abs(float(a["rc64"]) - 1.0) <= 1e-12
+
+
 # ------------------------------------------------- learned-jump MLP oracle ----
 
 def test_mlp_oracle_gradient_finite_differences():
