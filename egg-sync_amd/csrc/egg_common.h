@@ -164,7 +164,42 @@ struct DevState {
     int unit_next;
     int ndt;
     int ptile_next;    // work-queue head of the pair update (k_update_pairs)
+    // run_start[0 .. runs] describes runs of bitwise-equal positions in the
+    // current order of the positions (set by k_runs; an update moves every
+    // point of a run alike, so the runs survive it and the next grid build
+    // sorts runs instead of points)
+    int runs_valid;
+    int sort_m;        // elements of the grid build's sort: runs, or n
+    int sort_hist;     // 256 x the sort tiles holding them (histogram entries)
+    int lens_m;        // runs whose lengths are scanned (0 without valid runs)
 };
+
+// Start of a pass: the local-sync flag and the work-queue heads.
+__device__ __forceinline__ void reset_pass_state(DevState *st) {
+    st->flag = 1;
+    st->pair_count = 0;
+    st->tile_next = 0;
+    st->xtile_next = 0;
+    st->heavy_pack = 0;
+    st->unit_next = 0;
+    st->ndt = 0;
+    st->ptile_next = 0;
+}
+
+// End of an iteration (EGG_SynC.cu:1322,1425 / SynC.cpp:277): count it and
+// apply the stop rule and the iteration cap.
+__device__ __forceinline__ void finish_iteration(DevState *st) {
+    st->iter += 1;
+    if (st->stop_rule == 0) {
+        if (st->flag) st->done = 1;
+    } else {
+        if (st->rc >= (double)st->lam) st->done = 1;
+    }
+    if (!st->done && st->max_iter > 0 && st->iter >= st->max_iter) {
+        st->done = 1;
+        st->capped = 1;
+    }
+}
 
 // ---------------------------------------------------------------- helpers --
 

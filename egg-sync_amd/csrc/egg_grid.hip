@@ -58,6 +58,16 @@ void launch_fill(int *p, int count, int v, const DevState *st, hipStream_t s) {
     if (count > 0) k_fill<<<grid_for(count), kBlock, 0, s>>>(p, count, v, st);
 }
 
+// Zeroes two arrays of `count` ints in one launch (the outer-cell ranges).
+__global__ void k_zero2(int *__restrict__ p, int *__restrict__ q, int count, const DevState *st) {
+    if (st && st->done) return;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x) {
+        p[i] = 0;
+        q[i] = 0;
+    }
+}
+
 // Key bit fields: [outer | g_od | ... | g_{d-1} | sub_0 ... sub_{d-1} | hash],
 // least significant last, so numeric key order = lexicographic (outer, g_od,
 // ..., g_{d-1}, sub_0, ..., sub_{d-1}, position hash); the split bits and the
@@ -72,12 +82,27 @@ __device__ __forceinline__ void put_bits(uint64_t (&w)[kMaxKW], uint64_t v, int 
     }
 }
 
+// Sort keys of the grid build, one per element: per run of the previous
+// grid when its runs are valid (every point of a run has the run's key, and
+// runs are contiguous in X's order, so a stable sort of the runs expands to
+// the stable sort of the points), else per point.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_keys(const float *__restrict__ X, Geo g,
-                                                 uint64_t *__restrict__ keys, DevState *st_loop) {
+                                                 const int *__restrict__ run_start,
+                                                 uint64_t *__restrict__ keys, DevState *st,
+                                                 DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int n = g.n;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const bool valid = st->runs_valid != 0;
+    const int m = valid ? st->runs : n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->sort_m = m;
+        st->sort_hist = 256 * max(1, (m + kSortTile - 1) / kSortTile);
+        // m == n: every run is one point, run r is point r, as in a point sort
+        st->lens_m = m < n ? m : 0;
+    }
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+        const int i = valid ? run_start[e] : e;
         int c[D];
         float x[D];
 #pragma unroll
@@ -117,11 +142,25 @@ __global__ __launch_bounds__(kBlock) void k_keys(const float *__restrict__ X, Ge
         put_bits(kw, outer, shift, g.bits_o);
 #pragma unroll
         for (int k = 0; k < kMaxKW; k++)
-            if (k < g.kw) keys[(long long)k * n + i] = kw[k];
+            if (k < g.kw) keys[(long long)k * n + e] = kw[k];
     }
 }
 
-// Whether sorted keys i and i-1 agree above their low `s` bits.
+// Lengths of the sorted runs (sorted position q holds run sidx[q]), for the
+// scan that places each run's points.
+__global__ __launch_bounds__(kBlock) void k_run_lens(const int *__restrict__ sidx,
+                                                     const int *__restrict__ run_start,
+                                                     const DevState *st, int *__restrict__ lens,
+                                                     const DevState *st_loop) {
+    if (st_loop && st_loop->done) return;
+    const int m = st->lens_m;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x) {
+        const int r = sidx ? sidx[q] : q;
+        lens[q] = run_start[r + 1] - run_start[r];
+    }
+}
+
+// Whether sorted keys i and i-1 (stride n) agree above their low `s` bits.
 __device__ __forceinline__ bool same_above(const uint64_t *__restrict__ skeys, int n, int i,
                                            int kw, int s) {
     const int q = s >> 6, o = s & 63;
@@ -142,35 +181,64 @@ __device__ __forceinline__ bool same_above(const uint64_t *__restrict__ skeys, i
 
 // Sorted positions and perm, and three head flags per sorted position: cell
 // head, sub-cell head, run head (the position differs bitwise from its
-// predecessor's; equal positions always share a cell and a sub-cell).
+// predecessor's; equal positions always share a cell and a sub-cell).  The
+// sort ordered st->sort_m elements: runs of the previous grid (st->lens_m >
+// 0; sorted run q holds run sidx[q], its points go to [off[q], off[q] +
+// length) in their previous order) or points.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_gather(
     const float *__restrict__ X, const int *__restrict__ perm, const uint64_t *__restrict__ skeys,
-    const int *__restrict__ sidx, Geo g, float *__restrict__ Xs, int *__restrict__ perm_s,
-    int *__restrict__ head, int *__restrict__ shead, int *__restrict__ rhead,
+    const int *__restrict__ sidx, const int *__restrict__ run_start,
+    const int *__restrict__ off, Geo g, float *__restrict__ Xs, int *__restrict__ perm_s,
+    int *__restrict__ head, int *__restrict__ shead, int *__restrict__ rhead, const DevState *st,
     DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int n = g.n, sb = g.sub_bits + g.hash_bits, hb = g.hash_bits;
+    const int m = st->lens_m;  // 0: the sort ordered points
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int j = sidx ? sidx[i] : i;
-        const int jp = i > 0 ? (sidx ? sidx[i - 1] : i - 1) : j;
+        // sorted element q holding slot i, the slot's source j in X, and
+        // whether i starts q (only then can it start a cell, sub-cell or run)
+        int q = i, j, jp;
+        bool first = true;
+        if (m == 0) {
+            j = sidx ? sidx[i] : i;
+            jp = i > 0 ? (sidx ? sidx[i - 1] : i - 1) : j;
+        } else {
+            if (m < n) {
+                int lo = 0, hi = m - 1;  // the last q with off[q] <= i
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (off[mid] <= i) lo = mid;
+                    else hi = mid - 1;
+                }
+                q = lo;
+            }
+            const int r = sidx ? sidx[q] : q;
+            const int o = m < n ? off[q] : q;
+            first = i == o;
+            j = run_start[r] + (i - o);
+            jp = j;
+            if (first && q > 0) jp = run_start[sidx ? sidx[q - 1] : q - 1];
+        }
         bool rh = i == 0;
 #pragma unroll
         for (int l = 0; l < D; l++) {
             const float x = X[(long long)l * n + j];
             Xs[(long long)l * n + i] = x;
-            rh |= __float_as_uint(x) != __float_as_uint(X[(long long)l * n + jp]);
+            if (first) rh |= __float_as_uint(x) != __float_as_uint(X[(long long)l * n + jp]);
         }
         perm_s[i] = perm[j];
         int h = 1, hs = 1;
-        if (i > 0) {
+        if (!first) {
+            h = hs = 0;
+        } else if (q > 0) {
             // the sub-cell part: the key without its low hash_bits bits;
             // the cell part: without the split bits either
-            hs = !same_above(skeys, n, i, g.kw, hb);
-            h = !same_above(skeys, n, i, g.kw, sb);
+            hs = !same_above(skeys, n, q, g.kw, hb);
+            h = !same_above(skeys, n, q, g.kw, sb);
         }
-        head[i] = h;
-        if (shead) shead[i] = hs;
+        head[i] = i == 0 ? 1 : h;
+        if (shead) shead[i] = i == 0 ? 1 : hs;
         rhead[i] = rh ? 1 : 0;
     }
 }
@@ -258,21 +326,27 @@ __global__ __launch_bounds__(kBlock) void k_outer_ranges(const int *__restrict__
 void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
     const int n = g.n;
     const int gb = grid_for(n);
-#define KEYS_CASE(DD) \
-    case DD: k_keys<DD><<<gb, kBlock, 0, s>>>(b.X, g, b.keys_a, st_loop); break;
+#define KEYS_CASE(DD)                                                                        \
+    case DD:                                                                                 \
+        k_keys<DD><<<gb, kBlock, 0, s>>>(b.X, g, b.run_start, b.keys_a, st, st_loop); break;
     switch (g.d) {
         EGG_EACH_D(KEYS_CASE)
     }
 #undef KEYS_CASE
     int where = radix_sort(b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, g.kw, g.sort_bits, b.hist,
-                           b.scan_tmp, st_loop, s, true);
+                           b.scan_tmp, st_loop, s, true, &st->sort_m, &st->sort_hist);
     const uint64_t *skeys = where == 1 ? b.keys_b : b.keys_a;
     const int *sidx = where == 1 ? b.vals_b : (where == 0 ? b.vals_a : nullptr);
-#define GATHER_CASE(DD)                                                                  \
-    case DD:                                                                             \
-        k_gather<DD><<<gb, kBlock, 0, s>>>(b.X, b.perm, skeys, sidx, g, b.Xs, b.perm_s,  \
-                                           b.head, g.sub_bits ? b.shead : nullptr,      \
-                                           b.rhead, st_loop);                           \
+    // where the sorted runs' points go (nchunk / wi_off are free until the
+    // summaries)
+    k_run_lens<<<gb, kBlock, 0, s>>>(sidx, b.run_start, st, b.nchunk, st_loop);
+    scan_exclusive(b.nchunk, b.wi_off, n, &st->lens_m, b.scan_tmp, nullptr, st_loop, s, true);
+#define GATHER_CASE(DD)                                                                     \
+    case DD:                                                                                \
+        k_gather<DD><<<gb, kBlock, 0, s>>>(b.X, b.perm, skeys, sidx, b.run_start, b.wi_off, \
+                                           g, b.Xs, b.perm_s, b.head,                       \
+                                           g.sub_bits ? b.shead : nullptr, b.rhead, st,     \
+                                           st_loop);                                        \
         break;
     switch (g.d) {
         EGG_EACH_D(GATHER_CASE)
@@ -289,8 +363,7 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
         ss.out[2] = b.spos;
         scan_exclusive_multi(ss, g.sub_bits ? 3 : 2, n, nullptr, b.scan_tmp, st_loop, s, true);
     }
-    launch_fill(b.ob, g.O, 0, st_loop, s);
-    launch_fill(b.oe, g.O, 0, st_loop, s);
+    if (g.O > 0) k_zero2<<<grid_for(g.O), kBlock, 0, s>>>(b.ob, b.oe, g.O, st_loop);
 #define CELLS_CASE(DD)                                                                      \
     case DD:                                                                                \
         k_cells<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.head, b.hpos, g, b.point_cell,            \
@@ -351,6 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_runs(const int *__restrict__ head,
             run_start[r + 1] = n;
             cell_run[c + 1] = r + 1;
             st->runs = r + 1;
+            st->runs_valid = 1;
         }
     }
 }
@@ -559,15 +633,6 @@ __global__ __launch_bounds__(kBlock) void k_summary_combine(
     }
 }
 
-// Total of an exclusive scan of *count values (summary work items per cell,
-// update tiles per group).
-__global__ void k_store_total(const int *__restrict__ off, const int *__restrict__ cnt,
-                              const int *count, int *out, DevState *st_loop) {
-    if (st_loop && st_loop->done) return;
-    const int m = *count;
-    *out = m > 0 ? off[m - 1] + cnt[m - 1] : 0;
-}
-
 // The runs of the groups sparse_group selects, listed for k_update_pairs at
 // the offsets of the scan of their counts: one thread per run (its group is
 // gpos[c] + gflag[c] - 1 of its cell c).
@@ -611,8 +676,7 @@ static void group_summaries(const Geo &g, GridBufs &b, const int *starts, const 
     const int n = g.n;
     const int gb = grid_for(n);
     k_wi_count<<<gb, kBlock, 0, s>>>(starts, count, nchunk, st_loop);
-    scan_exclusive(nchunk, wi_off, n, count, b.scan_tmp, nullptr, st_loop, s, true);
-    k_store_total<<<1, 1, 0, s>>>(wi_off, nchunk, count, nwork, st_loop);
+    scan_exclusive(nchunk, wi_off, n, count, b.scan_tmp, nwork, st_loop, s, true);
     // work items <= groups + n / kSumChunk <= n + n / kSumChunk
     const long long max_wi = (long long)n + n / kSumChunk + 1;
     const int wb = grid_for(max_wi * kWave, kBlock, 16384);

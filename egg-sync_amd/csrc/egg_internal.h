@@ -34,12 +34,16 @@ void scan_exclusive_multi(const ScanSet &ss, int nch, int n, const int *n_dev, i
 
 // Stable LSD radix sort of (key, position) pairs by key bits [0, key_bits),
 // 8 bits per pass.  keys_a/keys_b: [kw][n] u64; vals_a/vals_b: [n] int32.
-// On entry keys are in keys_a and vals are implicit (0..n-1).  Returns 0 if
-// the sorted result is in the *_a buffers, 1 if in *_b.
+// On entry keys are in keys_a and vals are implicit (0..m-1); m = *n_dev
+// (<= n) if n_dev is given, else n.  With n_dev, *hist_n_dev must hold
+// 256 * max(1, ceil(m / kSortTile)) (the histogram entries of the tiles that
+// hold elements).  Returns 0 if the sorted result is in the *_a buffers, 1 if
+// in *_b, -1 if no pass was needed (identity order).
 size_t sort_tmp_ints(int n);
 int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int n, int kw,
                int key_bits, int *hist, int *scan_tmp, const DevState *st, hipStream_t s,
-               bool lookback = false);
+               bool lookback = false, const int *n_dev = nullptr,
+               const int *hist_n_dev = nullptr);
 
 // A heavy streamed range of an update tile: tile t, candidate cell c, the
 // range's runs (runs [r0, r0 + total) if smask is 0, else the runs of

@@ -214,6 +214,31 @@ __global__ __launch_bounds__(kBlock) void k_scan_lookback(ScanSet ss, int n, con
     __shared__ int s_tile, s_pre[NC];
     unsigned long long *status = (unsigned long long *)(lb + 4);
     const int nb = gridDim.x;
+    if (n_dev && *n_dev <= kScanTile) {
+        // one tile at run time: workgroup 0 scans it alone (no look-back
+        // state is touched), the others return
+        if (blockIdx.x != 0) return;
+        const int m = min(n, max(0, *n_dev));
+        const int base = threadIdx.x * kScanItems;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            int vals[kScanItems], v = 0;
+#pragma unroll
+            for (int k = 0; k < kScanItems; k++) {
+                vals[k] = base + k < m ? ss.in[c][base + k] : 0;
+                v += vals[k];
+            }
+            int tot;
+            int e = block_excl_scan(v, s_w[c], tot);
+#pragma unroll
+            for (int k = 0; k < kScanItems; k++) {
+                if (base + k < m) ss.out[c][base + k] = e;
+                e += vals[k];
+            }
+            if (threadIdx.x == 0 && ss.total[c]) *ss.total[c] = tot;
+        }
+        return;
+    }
     if (threadIdx.x == 0) s_tile = atomicAdd(&lb[0], 1);
     __syncthreads();
     const int tile = s_tile;
@@ -349,35 +374,47 @@ __device__ __forceinline__ int digit_of(const uint64_t *keys, int n, int e, int 
     return (int)((w >> (shift & 63)) & 255ull);
 }
 
+// Keys are stored with stride n ([kw][n]); the first m = min(n, *n_dev) of
+// them (all n without n_dev) are sorted.
 __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint64_t *__restrict__ keys, int n,
-                                                    int shift, int *__restrict__ hist,
-                                                    int ntiles, const DevState *st) {
+                                                    const int *n_dev, int shift,
+                                                    int *__restrict__ hist, int ntiles,
+                                                    const DevState *st) {
     if (st && st->done) return;
     __shared__ int h[256];
+    const int m = n_dev ? min(n, *n_dev) : n;
+    // tiles holding elements: the histogram's stride (its scan covers 256
+    // of them per digit)
+    const int nt = n_dev ? max(1, (m + kSortTile - 1) / kSortTile) : ntiles;
+    if ((int)blockIdx.x >= nt) return;
     h[threadIdx.x] = 0;
     __syncthreads();
     const long long base = (long long)blockIdx.x * kSortTile;
 #pragma unroll
     for (int k = 0; k < kSortItems; k++) {
         long long e = base + k * kBlock + threadIdx.x;
-        if (e < n) atomicAdd(&h[digit_of(keys, n, (int)e, shift)], 1);
+        if (e < m) atomicAdd(&h[digit_of(keys, n, (int)e, shift)], 1);
     }
     __syncthreads();
-    hist[(long long)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    hist[(long long)threadIdx.x * nt + blockIdx.x] = h[threadIdx.x];
 }
 
 template <int KW>
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint64_t *__restrict__ kin,
                                                        const int *__restrict__ vin,
                                                        uint64_t *__restrict__ kout,
-                                                       int *__restrict__ vout, int n, int shift,
+                                                       int *__restrict__ vout, int n,
+                                                       const int *n_dev, int shift,
                                                        const int *__restrict__ hist_scan,
                                                        int ntiles, const DevState *st) {
     if (st && st->done) return;
+    const int m = n_dev ? min(n, *n_dev) : n;
+    const int nt = n_dev ? max(1, (m + kSortTile - 1) / kSortTile) : ntiles;
+    if ((int)blockIdx.x >= nt) return;
     __shared__ int s_off[256];
     __shared__ int s_wc[kBlock / kWave][256];
     const int t = threadIdx.x, wid = t / kWave;
-    s_off[t] = hist_scan[(long long)t * ntiles + blockIdx.x];
+    s_off[t] = hist_scan[(long long)t * nt + blockIdx.x];
 #pragma unroll
     for (int w = 0; w < kBlock / kWave; w++) s_wc[w][t] = 0;
     __syncthreads();
@@ -385,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint64_t *__restric
     const int word = shift >> 6, sh = shift & 63;
     for (int k = 0; k < kSortItems; k++) {
         const long long e = base + k * kBlock + t;
-        const bool valid = e < n;
+        const bool valid = e < m;
         uint64_t kv[KW];
         int val = 0, dig = 0;
         if (valid) {
@@ -434,7 +471,7 @@ size_t sort_tmp_ints(int n) {
 
 int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int n, int kw,
                int key_bits, int *hist, int *scan_tmp, const DevState *st, hipStream_t s,
-               bool lookback) {
+               bool lookback, const int *n_dev, const int *hist_n_dev) {
     int ntiles = (n + kSortTile - 1) / kSortTile;
     if (ntiles == 0) ntiles = 1;
     const int passes = (key_bits + 7) / 8;
@@ -443,12 +480,12 @@ int radix_sort(uint64_t *keys_a, uint64_t *keys_b, int *vals_a, int *vals_b, int
     int where = 0;
     for (int p = 0; p < passes; p++) {
         const int shift = p * 8;
-        k_rs_hist<<<ntiles, kBlock, 0, s>>>(kin, n, shift, hist, ntiles, st);
-        scan_exclusive(hist, hist, 256 * ntiles, nullptr, scan_tmp, nullptr, st, s, lookback);
+        k_rs_hist<<<ntiles, kBlock, 0, s>>>(kin, n, n_dev, shift, hist, ntiles, st);
+        scan_exclusive(hist, hist, 256 * ntiles, hist_n_dev, scan_tmp, nullptr, st, s, lookback);
 #define RS_CASE(KW)                                                                          \
     case KW:                                                                                 \
-        k_rs_scatter<KW><<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, hist,     \
-                                                   ntiles, st);                              \
+        k_rs_scatter<KW><<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, n_dev, shift,    \
+                                                   hist, ntiles, st);                        \
         break;
         switch (kw) {
             RS_CASE(1) RS_CASE(2) RS_CASE(3) RS_CASE(4)

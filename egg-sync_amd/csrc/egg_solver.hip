@@ -338,9 +338,11 @@ __global__ void k_snapshot_flag(DevState *st) {
     st->flag_local = st->flag;
 }
 
-__global__ void k_trace(const DevState *st, const int *it_before, double *rc_trace,
-                        int *cells_trace, int cap) {
-    // after k_finish: record r_c and C of the iteration that just ran
+// End of a loop iteration: finish_iteration, then record r_c and C of the
+// iteration that just ran.
+__global__ void k_iter_end(DevState *st, const int *it_before, double *rc_trace,
+                           int *cells_trace, int cap) {
+    if (!st->done) finish_iteration(st);
     const int it = st->iter;
     if (it == *it_before) return;  // the loop had already stopped
     if (it >= 1 && it <= cap) {
@@ -349,8 +351,11 @@ __global__ void k_trace(const DevState *st, const int *it_before, double *rc_tra
     }
 }
 
-__global__ void k_iter_snapshot(const DevState *st, int *it_before) {
+// Start of a loop iteration: the iteration count before it, then the pass
+// state reset (nothing once the loop has stopped).
+__global__ void k_iter_begin(DevState *st, int *it_before) {
     *it_before = st->iter;
+    if (!st->done) reset_pass_state(st);
 }
 
 int grid_blocks(long long n) {
@@ -433,8 +438,7 @@ double ms_between(hipEvent_t a, hipEvent_t b) {
 void enqueue_iteration(const Geo &g, Layout &L, const egg_sync_options &opt, hipStream_t s,
                        hipEvent_t *ev) {
     if (ev) (void)hipEventRecord(ev[0], s);
-    k_iter_snapshot<<<1, 1, 0, s>>>(L.st, L.it_before);
-    launch_set_flag(L.st, s);
+    k_iter_begin<<<1, 1, 0, s>>>(L.st, L.it_before);
     build_grid(g, L.b, L.st, L.st, s);
     if (opt.stop_rule == EGG_SYNC_STOP_EGG) cell_summaries(g, L.b, L.st, L.st, s);
     if (ev) (void)hipEventRecord(ev[1], s);
@@ -447,8 +451,7 @@ void enqueue_iteration(const Geo &g, Layout &L, const egg_sync_options &opt, hip
         launch_update_sync(g, L.b, L.st, L.rc_part, true, s);
         if (ev) (void)hipEventRecord(ev[2], s);
     }
-    launch_finish(L.st, s);
-    k_trace<<<1, 1, 0, s>>>(L.st, L.it_before, L.rc_trace, L.cells_trace, kTraceCap);
+    k_iter_end<<<1, 1, 0, s>>>(L.st, L.it_before, L.rc_trace, L.cells_trace, kTraceCap);
     if (ev) (void)hipEventRecord(ev[3], s);
 }
 

@@ -1717,29 +1717,12 @@ __global__ __launch_bounds__(kBlock) void k_rc_finish(const double *__restrict__
 
 __global__ void k_set_flag(DevState *st) {
     if (st->done) return;
-    st->flag = 1;
-    st->pair_count = 0;
-    st->tile_next = 0;
-    st->xtile_next = 0;
-    st->heavy_pack = 0;
-    st->unit_next = 0;
-    st->ndt = 0;
-    st->ptile_next = 0;
+    reset_pass_state(st);
 }
 
-// End of an iteration (EGG_SynC.cu:1322,1425 / SynC.cpp:277).
 __global__ void k_finish(DevState *st) {
     if (st->done) return;
-    st->iter += 1;
-    if (st->stop_rule == 0) {
-        if (st->flag) st->done = 1;
-    } else {
-        if (st->rc >= (double)st->lam) st->done = 1;
-    }
-    if (!st->done && st->max_iter > 0 && st->iter >= st->max_iter) {
-        st->done = 1;
-        st->capped = 1;
-    }
+    finish_iteration(st);
 }
 
 static inline int grid_for(long long n) {
