@@ -709,6 +709,11 @@ int setup_single(const float *d_D, int64_t n64, int d, float eps, const int32_t 
 // iterations of all streams are captured once, forked from and joined to one
 // origin stream, and the graph is replayed.
 
+// Kernel nodes of one captured loop iteration (an upper bound) and the
+// largest batched graph that is captured.
+constexpr int kGraphNodesPerIter = 64;
+constexpr long long kMaxBatchGraphNodes = 65536;
+
 struct BatchWs {
     void *base = nullptr;
     size_t bytes = 0;
@@ -909,10 +914,18 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
         return join_streams(bw, origin, S);
     };
     // the graph holds the live set it was captured with; it is captured again
-    // when that set has shrunk to half, and plain launches cover the rest
+    // when that set has shrunk to half, and plain launches cover the rest.
+    // Graphs are kept to kMaxBatchGraphNodes kernel nodes: a capture of 256
+    // instances x 16 iterations (~200k nodes) ended the host process with a
+    // segmentation fault, 256 x 4 (~80k) did not.
     int graph_live = 0;
     auto ensure_graph = [&](int n_live) -> int {
         if (!opt.use_graph) return 0;
+        if ((long long)n_live * K * kGraphNodesPerIter > kMaxBatchGraphNodes) {
+            drop_graph(bw.gexec, bw.gkey);
+            graph_live = 0;
+            return 0;
+        }
         if (bw.gexec && graph_live > 0 && 2 * n_live > graph_live) return 0;
         std::vector<char> key;
         key_add(key, bw.base);

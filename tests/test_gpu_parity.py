@@ -319,6 +319,75 @@ def test_full_size_config1_against_fp64_oracle(S, O):
     assert err <= 1e-5 and drc <= 1e-6
 
 
+def test_full_size_config4(S, O):
+    """n = 4,000,000, d = 3, eps = 0.01 against the oracle's full solve
+    (tests/golden/c4_full_oracle.npz, 24 passes) and the oracle run from the
+    GPU's own states.
+
+    The GPU solve is deterministic and converges after 20 passes to 10,344
+    clusters, the oracle's count; its partition differs from the oracle's in
+    a few dozen of the 4M points.  The difference comes from the dense passes'
+    FP32 sums, not from a pass that decides differently: from any state both
+    take the same integer decisions (sorted order, neighbour counts, eps/2
+    counts, border and sync flags, both flags of the pass), and the oracle
+    continued from the GPU's state after 14 passes converges after the same
+    total number of passes, to the same partition.  The FP32 oracle adds the
+    points of a run of equal positions one by one (its collapsed clusters
+    hold up to ~10^5 points), the GPU weights the run's term by its length;
+    against an FP64 pass the GPU is the closer of the two."""
+    from egg_sync import data
+    from egg_sync.metrics import disagreement
+    X, eps = data.config4()
+    g = golden("c4_full_oracle")
+    n = len(X)
+    _say("config4 full solve")
+    a = S.solve(X, eps)
+    it = a["stats"]["iterations"]
+    assert a["stats"]["status"] == 0
+    b = S.solve(X, eps)
+    assert np.array_equal(a["X"], b["X"]) and np.array_equal(a["labels"], b["labels"])
+    dis = disagreement(a["labels"], g["labels"])
+    _say("config4: gpu", it, "passes,", a["stats"]["n_clusters"], "clusters; oracle",
+         int(g["iterations"]), "passes; disagreement", dis)
+    assert a["stats"]["n_clusters"] == int(g["labels"].max()) + 1
+    assert dis["noise_differs"] + dis["unmatched"] <= 1e-4 * n
+    assert O.same_partition(a["labels"], O.egg_labels(a["X"], eps))
+
+    def pass_equal(Xk, tag):
+        ga = S.iteration(Xk, eps)
+        gb = O.egg_iteration(Xk, eps)
+        for k in ("order", "N", "half", "border", "synced"):
+            assert np.array_equal(ga[k], gb[k]), (tag, k)
+        assert (ga["flag_local"], ga["flag_final"]) == (gb["flag_local"], gb["flag_final"]), tag
+        return ga, gb
+
+    # after 13 passes (clusters collapsing into long runs): integer outputs
+    # bit-exact; positions per point no further from an FP64 pass than the
+    # FP32 oracle's, plus 1e-6
+    X13 = S.solve(X, eps, max_iter=13)["X"]
+    _say("config4 pass from the state after 13 passes")
+    ga, gb = pass_equal(X13, "13")
+    x64 = O.egg_solve(X13, eps, max_iter=1, fp64=True)["X"].astype(np.float64)
+    e_gpu = np.abs(ga["X_next"].astype(np.float64) - x64).max(axis=1)
+    e_o32 = np.abs(gb["X_next"].astype(np.float64) - x64).max(axis=1)
+    _say("  max |x - x_fp64|: gpu", e_gpu.max(), "fp32 oracle", e_o32.max())
+    assert np.all(e_gpu <= e_o32 + 1e-6), float((e_gpu - e_o32).max())
+    # the oracle continued from the state after 14 passes
+    cur = S.solve(X, eps, max_iter=14)["X"]
+    pass_equal(cur, "14")
+    order, more = None, 0
+    while more < 3 * it:
+        r = O.egg_iteration(cur, eps, order)
+        cur, order, more = r["X_next"], r["order"], more + 1
+        if r["flag_final"]:
+            break
+    _say("  oracle from the gpu state after 14 passes: converges after", 14 + more)
+    assert 14 + more == it
+    assert O.same_partition(O.egg_labels(cur, eps), a["labels"])
+    # the converged state
+    pass_equal(a["X"], "final")
+
+
 def test_config1_sample_solve_matches_oracle(S, O):
     from egg_sync import data
     X, eps = data.config1(n=100_000)
