@@ -132,7 +132,7 @@ int egg_sync_solve_device(const float *d_D, int64_t n, int32_t d, float eps,
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
  * workspace is a 64 MiB device array inside the library, one per GPU, held
  * until the process ends; a workload whose workspace exceeds it returns
- * EGG_SYNC_E_TOO_BIG (about n <= 145,000 at d = 3, 90,000 at d = 8;
+ * EGG_SYNC_E_TOO_BIG (about n <= 135,000 at d = 3, 70,000 at d = 8;
  * stats_out->workspace_bytes reports the bytes a call used).  The caller
  * passes inputs and outputs as for egg_sync_solve_device.  No graph, no
  * timing events (use_graph and timing are ignored), the caller's stream or
