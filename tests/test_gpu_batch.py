@@ -254,3 +254,60 @@ def test_launch_shapes_bitwise_neutral_at_full_size(S):
     for qt, qb in ((64, 0), (128, 512)):
         _say("config1 queue shape", qt, qb)
         _same(ref, S.solve(X, eps, queue_threads=qt, queue_blocks=qb))
+
+
+def test_config3_slowest_instance_where_trajectories_part(S, O):
+    """Why the slowest config-3 instance may stop a pass earlier or later than
+    the oracle: from the GPU's state after 8,607 passes the GPU and the oracle
+    step side by side, each from its own previous state.  Every pass takes the
+    same per-point decisions (N, half, border, synced, both flags) until a
+    cluster of ~20,000 equal points drifting across a cell face enters the
+    reference's +-1e-5 border band (EGG_SynC.cu:590-595) at different passes:
+    the two trajectories' positions part by the FP32 oracle's rounding (it adds
+    the cluster's equal terms one by one; the GPU weights the run by its
+    length), 3.3e-5 after ~2,600 passes.  No eps or eps/2 test decides
+    differently first: N is equal at the first differing pass, and every point
+    whose border flag differs has the band's edge between its two positions."""
+    from egg_sync import data
+    f32 = np.float32
+    X = data.config3_instances(seed=3, count=64)[27]
+    eps = 0.1
+    K = 8607
+    cs = float(f32(np.sqrt((float(f32(eps)) / 2) ** 2 / 2)))
+    X0 = S.solve(X, eps, max_iter=K)["X"]
+    _say("slowest instance: GPU and oracle from the GPU state after", K, "passes")
+
+    def per_point(r, f):
+        a = np.empty_like(r[f])
+        a[r["order"]] = r[f]
+        return a
+
+    def band_margin(x):
+        # distance of the nearest coordinate to the edge of a face's +-1e-5 band
+        xd = x.astype(np.float64)
+        face = np.round(xd / cs) * cs
+        return (np.abs(np.abs(xd - face) - 1e-5)).min(axis=1)
+
+    Xg, og, Xo, oo = X0, None, X0, None
+    for k in range(4000):
+        rg = S.iteration(Xg, eps, og)
+        ro = O.egg_iteration(Xo, eps, oo)
+        diff = [f for f in ("N", "half", "border", "synced")
+                if not np.array_equal(per_point(rg, f), per_point(ro, f))]
+        flags_equal = (rg["flag_local"], rg["flag_final"]) == (ro["flag_local"], ro["flag_final"])
+        if diff or not flags_equal:
+            gap = float(np.abs(Xg - Xo).max())
+            _say(f"  first differing pass {K + k + 1}: {diff}, positions {gap:.3g} apart")
+            assert "N" not in diff and flags_equal
+            assert "border" in diff and gap <= 1e-4
+            pts = np.nonzero(per_point(rg, "border") != per_point(ro, "border"))[0]
+            _say(f"  {len(pts)} points change border status")
+            m = np.minimum(band_margin(Xg[pts]), band_margin(Xo[pts]))
+            # (1e-6: the faces' own FP32 placement, id * cs)
+            assert np.all(m <= np.abs(Xg[pts] - Xo[pts]).max(axis=1) + 1e-6)
+            return
+        if rg["flag_final"]:
+            _say(f"  both stop at pass {K + k + 1} with equal decisions throughout")
+            return
+        Xg, og, Xo, oo = rg["X_next"], rg["order"], ro["X_next"], ro["order"]
+    pytest.fail("neither trajectory stopped")
