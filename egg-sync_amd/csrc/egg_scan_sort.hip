@@ -1,8 +1,10 @@
 // Device-wide exclusive scan and stable LSD radix sort for gfx950.
 //
-// Scan: reduce-then-scan, 2048-element tiles (256 threads x 8 items), wave
-// scans with lane shuffles; arrays of one tile or less are scanned by one
-// workgroup in a single launch.  Radix sort: 8-bit digits, per-tile LDS histogram,
+// Scan: 2048-element tiles (256 threads x 8 items), wave scans with lane
+// shuffles; arrays of one tile or less are scanned by one workgroup in a
+// single launch, longer ones by a single-pass decoupled look-back scan (the
+// grid build) or reduce-then-scan in three launches (callers without
+// look-back state).  Radix sort: 8-bit digits, per-tile LDS histogram,
 // exclusive scan of the digit-major histogram, then a stable scatter in which
 // each wavefront ranks its 64 keys by digit with 8 ballots (lanes holding the
 // same digit form a 64-bit match mask).  Stability makes the grid build
@@ -45,8 +47,8 @@ __device__ __forceinline__ int block_excl_scan(int v, int *s_w, int &total) {
 }
 
 // Multi-channel scans: up to kMaxScanCh int32 arrays of the same length are
-// scanned by the same three launches (one pass over all of them), each
-// channel with its own block sums and total.
+// scanned by the same launches (one pass over all of them), each channel with
+// its own block sums (or look-back words) and total.  Reduce-then-scan first.
 template <int NC>
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(ScanSet ss, int n, const int *n_dev,
                                                         int *__restrict__ bsum, int nb,
