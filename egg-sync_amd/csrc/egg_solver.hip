@@ -232,6 +232,14 @@ static int reset_scan_state(const Geo &g, Layout &L, hipStream_t s) {
     return scan_tmp_reset(L.b.scan_tmp, (int)std::max((size_t)g.n, sort_tmp_ints(g.n)), s);
 }
 
+// EGG_SYNC_E_INTERNAL if a look-back scan on this workspace gave up waiting.
+static int check_scans(const Layout &L, hipStream_t s) {
+    int err = 0;
+    const int rc = scan_tmp_error(L.b.scan_tmp, s, &err);
+    if (rc) return rc;
+    return err ? EGG_SYNC_E_INTERNAL : 0;
+}
+
 #define HIP_TRY(x)                       \
     do {                                 \
         int rc_ = hip_err((x));          \
@@ -650,10 +658,8 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
     if (fe) (void)hipEventRecord(fe[2], s);
     HIP_TRY(hipStreamSynchronize(s));
     const DevState fin = *h_state;
-    int scan_err = 0;
-    rc = scan_tmp_error(L.b.scan_tmp, s, &scan_err);
+    rc = check_scans(L, s);
     if (rc) return rc;
-    if (scan_err) return EGG_SYNC_E_INTERNAL;
     if (so) {
         so->iterations = loop_state.iter;
         so->status = loop_state.capped ? EGG_SYNC_CAPPED : EGG_SYNC_OK;
@@ -1017,6 +1023,10 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
     HIP_TRY(hipMemcpyAsync(bw.h_states, bw.states, sizeof(DevState) * n_inst,
                            hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    for (int k : active) {
+        rc = check_scans(Ls[k], s);
+        if (rc) return rc;
+    }
     bool capped = false;
     int64_t clusters = 0, cells = 0;
     for (int k : active) {
@@ -1142,7 +1152,7 @@ int egg_sync_build_grid_device(const float *d_D, int64_t n, int32_t d, float eps
                           hipMemcpyDeviceToDevice));
     if (C_out) *C_out = hs.C;
     HIP_TRY(hipGetLastError());
-    return EGG_SYNC_OK;
+    return check_scans(L, s);
 }
 
 int egg_sync_iteration_device(const float *d_D, int64_t n, int32_t d, float eps,
@@ -1172,7 +1182,7 @@ int egg_sync_iteration_device(const float *d_D, int64_t n, int32_t d, float eps,
     HIP_TRY(hipGetLastError());
     if (flag_local) *flag_local = hs.flag_local;
     if (flag_final) *flag_final = hs.flag;
-    return EGG_SYNC_OK;
+    return check_scans(L, s);
 }
 
 int egg_sync_features_device(const float *d_D, const float *d_prev, int64_t n, int32_t d,
@@ -1191,7 +1201,7 @@ int egg_sync_features_device(const float *d_D, const float *d_prev, int64_t n, i
                                                    1.f / eps, d_feat);
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipGetLastError());
-    return EGG_SYNC_OK;
+    return check_scans(L, s);
 }
 
 int egg_sync_rc_device(const float *d_D, int64_t n, int32_t d, float eps, double *rc_out,
@@ -1209,7 +1219,7 @@ int egg_sync_rc_device(const float *d_D, int64_t n, int32_t d, float eps, double
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipGetLastError());
     if (rc_out) *rc_out = hs.rc;
-    return EGG_SYNC_OK;
+    return check_scans(L, s);
 }
 
 int egg_sync_geometry(int64_t n, int32_t d, float eps, int64_t *out5, float *cs_out) {
