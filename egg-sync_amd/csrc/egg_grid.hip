@@ -58,16 +58,6 @@ void launch_fill(int *p, int count, int v, const DevState *st, hipStream_t s) {
     if (count > 0) k_fill<<<grid_for(count), kBlock, 0, s>>>(p, count, v, st);
 }
 
-// Zeroes two arrays of `count` ints in one launch (the outer-cell ranges).
-__global__ void k_zero2(int *__restrict__ p, int *__restrict__ q, int count, const DevState *st) {
-    if (st && st->done) return;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
-         i += (long long)gridDim.x * blockDim.x) {
-        p[i] = 0;
-        q[i] = 0;
-    }
-}
-
 // Key bit fields: [outer | g_od | ... | g_{d-1} | sub_0 ... sub_{d-1} | hash],
 // least significant last, so numeric key order = lexicographic (outer, g_od,
 // ..., g_{d-1}, sub_0, ..., sub_{d-1}, position hash); the split bits and the
@@ -89,10 +79,16 @@ __device__ __forceinline__ void put_bits(uint64_t (&w)[kMaxKW], uint64_t v, int 
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_keys(const float *__restrict__ X, Geo g,
                                                  const int *__restrict__ run_start,
-                                                 uint64_t *__restrict__ keys, DevState *st,
-                                                 DevState *st_loop) {
+                                                 uint64_t *__restrict__ keys,
+                                                 int *__restrict__ ob, int *__restrict__ oe,
+                                                 DevState *st, DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int n = g.n;
+    // the outer cells' ranges start empty (k_outer_ranges fills the occupied ones)
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < g.O; o += gridDim.x * blockDim.x) {
+        ob[o] = 0;
+        oe[o] = 0;
+    }
     const bool valid = st->runs_valid != 0;
     const int m = valid ? st->runs : n;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -243,48 +239,18 @@ __global__ __launch_bounds__(kBlock) void k_gather(
     }
 }
 
-// Sub-cell table: first sorted position, fine coordinates 2g + split bit,
-// first sub-cell of each cell.
+// Cell table per sorted position: cell of each point, first position, grid
+// coordinates and outer id of each cell; with split bits (shead != nullptr)
+// also the sub-cell table: first sorted position, fine coordinates 2g + split
+// bit, first sub-cell of each cell, and the sub-cells' empty boxes.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_subcells(
-    const float *__restrict__ Xs, const int *__restrict__ head, const int *__restrict__ shead,
-    const int *__restrict__ spos, const int *__restrict__ point_cell, Geo g,
-    int *__restrict__ sub_start, int *__restrict__ sub_g, int *__restrict__ cell_sub,
-    unsigned *__restrict__ smin, unsigned *__restrict__ smax, DevState *st, DevState *st_loop) {
-    if (st_loop && st_loop->done) return;
-    const int n = g.n;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int s = spos[i] + shead[i] - 1;
-        if (shead[i]) {
-            sub_start[s] = i;
-#pragma unroll
-            for (int l = 0; l < D; l++) {
-                const float x = Xs[(long long)l * n + i];
-                const int gl = cell_coord(x, g.cs, g.w);
-                sub_g[(long long)l * n + s] = 2 * gl + (x >= cell_mid(gl, g.cs) ? 1 : 0);
-                // empty box, widened by the summary pass (k_summary)
-                smin[(long long)l * n + s] = 0xFFFFFFFFu;
-                smax[(long long)l * n + s] = 0u;
-            }
-            if (head[i]) cell_sub[point_cell[i]] = s;
-        }
-        if (i == n - 1) {
-            sub_start[s + 1] = n;
-            cell_sub[point_cell[i] + 1] = s + 1;
-            st->S = s + 1;
-        }
-    }
-}
-
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_cells(const float *__restrict__ Xs,
-                                                  const int *__restrict__ head,
-                                                  const int *__restrict__ hpos, Geo g,
-                                                  int *__restrict__ point_cell,
-                                                  int *__restrict__ cell_start,
-                                                  int *__restrict__ cell_g,
-                                                  int *__restrict__ cell_outer, DevState *st_loop,
-                                                  DevState *st) {
+__global__ __launch_bounds__(kBlock) void k_cells(
+    const float *__restrict__ Xs, const int *__restrict__ head, const int *__restrict__ hpos,
+    const int *__restrict__ shead, const int *__restrict__ spos, Geo g,
+    int *__restrict__ point_cell, int *__restrict__ cell_start, int *__restrict__ cell_g,
+    int *__restrict__ cell_outer, int *__restrict__ sub_start, int *__restrict__ sub_g,
+    int *__restrict__ cell_sub, unsigned *__restrict__ smin, unsigned *__restrict__ smax,
+    DevState *st_loop, DevState *st) {
     if (st_loop && st_loop->done) return;
     const int n = g.n;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -306,13 +272,61 @@ __global__ __launch_bounds__(kBlock) void k_cells(const float *__restrict__ Xs,
             cell_start[c + 1] = n;
             st->C = c + 1;
         }
+        if (!shead) continue;
+        const int sp = spos[i] + shead[i] - 1;
+        if (shead[i]) {
+            sub_start[sp] = i;
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                const float x = Xs[(long long)l * n + i];
+                const int gl = cell_coord(x, g.cs, g.w);
+                sub_g[(long long)l * n + sp] = 2 * gl + (x >= cell_mid(gl, g.cs) ? 1 : 0);
+                // empty box, widened by the summary pass (k_summary)
+                smin[(long long)l * n + sp] = 0xFFFFFFFFu;
+                smax[(long long)l * n + sp] = 0u;
+            }
+            if (h) cell_sub[c] = sp;
+        }
+        if (i == n - 1) {
+            sub_start[sp + 1] = n;
+            cell_sub[c + 1] = sp + 1;
+            st->S = sp + 1;
+        }
     }
 }
 
+// Whether cell c heads a group of the update tiles.  Cells are sorted by
+// (outer id, g_od, ..., g_{d-1}), so cells that share the outer cell and every
+// inner coordinate but the last are adjacent; such a cell starts a new group
+// when its last coordinate leaves the previous cell's block of gw cells.  With
+// no inner dimension (od == d) or gw <= 1 every cell is its own group.
+__device__ __forceinline__ bool group_head(const int *__restrict__ cell_g,
+                                           const int *__restrict__ cell_outer, const Geo &g,
+                                           int c) {
+    const int n = g.n, last = g.d - 1;
+    const bool grouped = g.gw > 1 && g.od < g.d;
+    bool h = c == 0 || !grouped;
+    if (!h) {
+        h = cell_outer[c] != cell_outer[c - 1];
+        for (int l = g.od; l < last && !h; l++)
+            h = cell_g[(long long)l * n + c] != cell_g[(long long)l * n + c - 1];
+        if (!h)
+            h = cell_g[(long long)last * n + c] / g.gw != cell_g[(long long)last * n + c - 1] / g.gw;
+    }
+    return h;
+}
+
+// Per cell: the cell ranges of the outer cells (ob / oe), the cell's
+// summary work items (nchunk, ceil(points / kSumChunk)) and whether it heads
+// a group of the update tiles (gflag, group_head).
 __global__ __launch_bounds__(kBlock) void k_outer_ranges(const int *__restrict__ cell_outer,
+                                                         const int *__restrict__ cell_start,
+                                                         const int *__restrict__ cell_g, Geo g,
                                                          const DevState *st,
                                                          int *__restrict__ ob,
                                                          int *__restrict__ oe,
+                                                         int *__restrict__ nchunk,
+                                                         int *__restrict__ gflag,
                                                          DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int C = st->C;
@@ -320,6 +334,9 @@ __global__ __launch_bounds__(kBlock) void k_outer_ranges(const int *__restrict__
         const int o = cell_outer[c];
         if (c == 0 || cell_outer[c - 1] != o) ob[o] = c;
         if (c == C - 1 || cell_outer[c + 1] != o) oe[o] = c + 1;
+        const int cnt = cell_start[c + 1] - cell_start[c];
+        nchunk[c] = (cnt + kSumChunk - 1) / kSumChunk;
+        gflag[c] = group_head(cell_g, cell_outer, g, c) ? 1 : 0;
     }
 }
 
@@ -328,7 +345,8 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
     const int gb = grid_for(n);
 #define KEYS_CASE(DD)                                                                        \
     case DD:                                                                                 \
-        k_keys<DD><<<gb, kBlock, 0, s>>>(b.X, g, b.run_start, b.keys_a, st, st_loop); break;
+        k_keys<DD><<<gb, kBlock, 0, s>>>(b.X, g, b.run_start, b.keys_a, b.ob, b.oe, st, st_loop); \
+        break;
     switch (g.d) {
         EGG_EACH_D(KEYS_CASE)
     }
@@ -363,47 +381,22 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
         ss.out[2] = b.spos;
         scan_exclusive_multi(ss, g.sub_bits ? 3 : 2, n, nullptr, b.scan_tmp, st_loop, s, true);
     }
-    if (g.O > 0) k_zero2<<<grid_for(g.O), kBlock, 0, s>>>(b.ob, b.oe, g.O, st_loop);
 #define CELLS_CASE(DD)                                                                      \
     case DD:                                                                                \
-        k_cells<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.head, b.hpos, g, b.point_cell,            \
-                                          b.cell_start, b.cell_g, b.cell_outer, st_loop, st); \
+        k_cells<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.head, b.hpos, g.sub_bits ? b.shead : nullptr, \
+                                          b.spos, g, b.point_cell, b.cell_start, b.cell_g,      \
+                                          b.cell_outer, b.sub_start, b.sub_g, b.cell_sub,       \
+                                          b.smin, b.smax, st_loop, st);                         \
         break;
     switch (g.d) {
         EGG_EACH_D(CELLS_CASE)
     }
 #undef CELLS_CASE
-    k_outer_ranges<<<gb, kBlock, 0, s>>>(b.cell_outer, st, b.ob, b.oe, st_loop);
-    if (g.sub_bits) {
-#define SUB_CASE(DD)                                                                          \
-    case DD:                                                                                  \
-        k_subcells<DD><<<gb, kBlock, 0, s>>>(b.Xs, b.head, b.shead, b.spos, b.point_cell, g,  \
-                                             b.sub_start, b.sub_g, b.cell_sub, b.smin,        \
-                                             b.smax, st, st_loop);                            \
-        break;
-        switch (g.d) {
-            EGG_EACH_D(SUB_CASE)
-        }
-#undef SUB_CASE
-    }
+    k_outer_ranges<<<gb, kBlock, 0, s>>>(b.cell_outer, b.cell_start, b.cell_g, g, st, b.ob, b.oe,
+                                         b.nchunk, b.gflag, st_loop);
 }
 
 // ------------------------------------------------------------- summaries --
-
-// Summaries work on "groups" of consecutive sorted points -- cells, or
-// sub-cells -- given by their start offsets, coordinates and a count in
-// device memory (*count groups, *nwork work items).
-__global__ __launch_bounds__(kBlock) void k_wi_count(const int *__restrict__ cell_start,
-                                                     const int *count,
-                                                     int *__restrict__ nchunk,
-                                                     DevState *st_loop) {
-    if (st_loop && st_loop->done) return;
-    const int C = *count;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
-        const int cnt = cell_start[c + 1] - cell_start[c];
-        nchunk[c] = (cnt + kSumChunk - 1) / kSumChunk;
-    }
-}
 
 // Run table from the run-head scan: run_start[r], cell_run[c] (first run of
 // each cell), run_start[runs] = n, cell_run[C] = runs.
@@ -426,34 +419,6 @@ __global__ __launch_bounds__(kBlock) void k_runs(const int *__restrict__ head,
             st->runs = r + 1;
             st->runs_valid = 1;
         }
-    }
-}
-
-// Group heads of the update tiles.  Cells are sorted by (outer id, g_od, ...,
-// g_{d-1}), so cells that share the outer cell and every inner coordinate but
-// the last are adjacent; such a cell starts a new group when its last
-// coordinate leaves the previous cell's block of gw cells.  With no inner
-// dimension (od == d) or gw <= 1 every cell is its own group.
-__global__ __launch_bounds__(kBlock) void k_group_heads(const int *__restrict__ cell_g,
-                                                        const int *__restrict__ cell_outer,
-                                                        Geo g, const DevState *st,
-                                                        int *__restrict__ gflag,
-                                                        DevState *st_loop) {
-    if (st_loop && st_loop->done) return;
-    const int C = st->C, n = g.n, d = g.d, last = g.d - 1;
-    const bool grouped = g.gw > 1 && g.od < g.d;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
-        bool h = c == 0 || !grouped;
-        if (!h) {
-            h = cell_outer[c] != cell_outer[c - 1];
-            for (int l = g.od; l < last && !h; l++)
-                h = cell_g[(long long)l * n + c] != cell_g[(long long)l * n + c - 1];
-            if (!h)
-                h = cell_g[(long long)last * n + c] / g.gw !=
-                    cell_g[(long long)last * n + c - 1] / g.gw;
-        }
-        gflag[c] = h ? 1 : 0;
-        (void)d;
     }
 }
 
@@ -659,23 +624,30 @@ __global__ __launch_bounds__(kBlock) void k_sub_runs(const int *__restrict__ sub
                                                      const int *__restrict__ rpos,
                                                      const DevState *st,
                                                      int *__restrict__ sub_run,
+                                                     int *__restrict__ snchunk,
                                                      DevState *st_loop) {
     if (st_loop && st_loop->done) return;
     const int S = st->S;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x)
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x) {
         sub_run[k] = rpos[sub_start[k]];
+        // the sub-cell's summary work items
+        snchunk[k] = (sub_start[k + 1] - sub_start[k] + kSumChunk - 1) / kSumChunk;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) sub_run[S] = st->runs;
 }
 
 // Kahan per-group totals (see k_summary) of the groups [starts[k], starts[k+1]),
 // k < *count, with anchors coords * g.cs.
+// Summaries work on "groups" of consecutive sorted points -- cells, or
+// sub-cells -- given by their start offsets, coordinates and a count in
+// device memory (*count groups, *nwork work items).  nchunk (work items per
+// group) is filled by earlier kernels (k_outer_ranges for cells, k_sub_runs
+// for sub-cells).
 static void group_summaries(const Geo &g, GridBufs &b, const int *starts, const int *coords,
                             int *nchunk, int *wi_off, const int *count, int *nwork, float *Ts,
                             float *Tc, unsigned *mnk, unsigned *mxk, DevState *st_loop,
                             hipStream_t s) {
     const int n = g.n;
-    const int gb = grid_for(n);
-    k_wi_count<<<gb, kBlock, 0, s>>>(starts, count, nchunk, st_loop);
     scan_exclusive(nchunk, wi_off, n, count, b.scan_tmp, nwork, st_loop, s, true);
     // work items <= groups + n / kSumChunk <= n + n / kSumChunk
     const long long max_wi = (long long)n + n / kSumChunk + 1;
@@ -702,7 +674,6 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
     // runs of equal positions (heads scanned by build_grid) and the tiles over them
     k_runs<<<gb, kBlock, 0, s>>>(b.rhead, b.rpos, b.point_cell, b.cell_start, n, b.run_start,
                                  b.cell_run, st, st_loop);
-    k_group_heads<<<gb, kBlock, 0, s>>>(b.cell_g, b.cell_outer, g, st, b.gflag, st_loop);
     scan_exclusive(b.gflag, b.gpos, n, &st->C, b.scan_tmp, &st->G, st_loop, s, true);
     k_group_cells<<<gb, kBlock, 0, s>>>(b.gflag, b.gpos, st, b.grp_cell, st_loop);
     k_group_counts<<<gb, kBlock, 0, s>>>(b.cell_run, b.grp_cell, st, b.ntile, b.untile,
@@ -727,7 +698,7 @@ void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, 
                                             b.sparse_runs, g.pair_mode, g.d, st_loop);
     if (g.sub_bits) {
         // sub-cell runs and totals (anchors on the half-side grid)
-        k_sub_runs<<<gb, kBlock, 0, s>>>(b.sub_start, b.rpos, st, b.sub_run, st_loop);
+        k_sub_runs<<<gb, kBlock, 0, s>>>(b.sub_start, b.rpos, st, b.sub_run, b.snchunk, st_loop);
         Geo gf = g;
         gf.cs = g.cs * 0.5f;
         group_summaries(gf, b, b.sub_start, b.sub_g, b.snchunk, b.swi_off, &st->S, &st->WS,

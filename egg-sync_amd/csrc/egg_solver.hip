@@ -341,11 +341,6 @@ __global__ void k_features(const float *__restrict__ Xs, const int *__restrict__
     }
 }
 
-__global__ void k_snapshot_flag(DevState *st) {
-    if (st->done) return;
-    st->flag_local = st->flag;
-}
-
 // End of a loop iteration: finish_iteration, then record r_c and C of the
 // iteration that just ran.
 __global__ void k_iter_end(DevState *st, const int *it_before, double *rc_trace,
@@ -453,7 +448,6 @@ void enqueue_iteration(const Geo &g, Layout &L, const egg_sync_options &opt, hip
     if (opt.stop_rule == EGG_SYNC_STOP_EGG) {
         launch_update_egg(g, L.b, L.st, nullptr, s);
         if (ev) (void)hipEventRecord(ev[2], s);
-        k_snapshot_flag<<<1, 1, 0, s>>>(L.st);
         launch_extra_check(g, L.b, L.st, s);
     } else {
         launch_update_sync(g, L.b, L.st, L.rc_part, true, s);
@@ -1171,7 +1165,6 @@ int egg_sync_iteration_device(const float *d_D, int64_t n, int32_t d, float eps,
     build_grid(g, L.b, L.st, L.st, s);
     cell_summaries(g, L.b, L.st, L.st, s);
     launch_update_egg(g, L.b, L.st, &dbg, s);
-    k_snapshot_flag<<<1, 1, 0, s>>>(L.st);
     launch_extra_check(g, L.b, L.st, s);
     if (d_next) launch_soa_to_aos_by_perm(L.b.X, L.b.perm, d_next, g.n, d, s);
     if (d_order_out)

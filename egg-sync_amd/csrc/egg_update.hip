@@ -1363,6 +1363,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter_runs(
     const int *__restrict__ rdbg, const int *__restrict__ perm_s, float *__restrict__ Xout,
     int *__restrict__ perm_out, int n, DevState *st, DebugOut dbg) {
     if (st->done) return;
+    // the local-sync flag as the update left it, before the extra check
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->flag_local = st->flag;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         perm_out[i] = perm_s[i];
         const int r = rpos[i] + rhead[i] - 1;
