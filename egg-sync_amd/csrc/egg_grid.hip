@@ -340,7 +340,8 @@ __global__ __launch_bounds__(kBlock) void k_outer_ranges(const int *__restrict__
     }
 }
 
-void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s) {
+void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s,
+                bool small) {
     const int n = g.n;
     const int gb = grid_for(n);
 #define KEYS_CASE(DD)                                                                        \
@@ -351,7 +352,9 @@ void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipS
         EGG_EACH_D(KEYS_CASE)
     }
 #undef KEYS_CASE
-    int where = radix_sort(b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, g.kw, g.sort_bits, b.hist,
+    const int where =
+        small ? small_sort(b.keys_a, b.keys_b, b.vals_b, n, g.kw, &st->sort_m, st_loop, s)
+              : radix_sort(b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, g.kw, g.sort_bits, b.hist,
                            b.scan_tmp, st_loop, s, true, &st->sort_m, &st->sort_hist);
     const uint64_t *skeys = where == 1 ? b.keys_b : b.keys_a;
     const int *sidx = where == 1 ? b.vals_b : (where == 0 ? b.vals_a : nullptr);

@@ -32,6 +32,13 @@ struct ScanSet {
 void scan_exclusive_multi(const ScanSet &ss, int nch, int n, const int *n_dev, int *tmp,
                           const DevState *st, hipStream_t s, bool lookback = false);
 
+// The same stable order from one workgroup comparing every pair of the first
+// *n_dev keys (for a few hundred keys, where the radix passes' launches
+// dominate); the result is in the *_b buffers (returns 1).
+constexpr int kSmallSortRuns = 1024;
+int small_sort(const uint64_t *keys_a, uint64_t *keys_b, int *vals_b, int n, int kw,
+               const int *n_dev, const DevState *st, hipStream_t s);
+
 // Stable LSD radix sort of (key, position) pairs by key bits [0, key_bits),
 // 8 bits per pass.  keys_a/keys_b: [kw][n] u64; vals_a/vals_b: [n] int32.
 // On entry keys are in keys_a and vals are implicit (0..m-1); m = *n_dev
@@ -132,7 +139,9 @@ void launch_soa_to_aos_by_perm(const float *soa, const int *perm, float *aos, in
                                hipStream_t s);
 void launch_fill(int *p, int count, int v, const DevState *st, hipStream_t s);
 // keys + sort + gather + cells + outer ranges.  Writes st->C.
-void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s);
+// small: the previous grid holds at most kSmallSortRuns runs (small_sort).
+void build_grid(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s,
+                bool small = false);
 void cell_summaries(const Geo &g, GridBufs &b, DevState *st_loop, DevState *st, hipStream_t s);
 
 // Update / extra check / r_c (egg_update.hip).

@@ -465,6 +465,52 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint64_t *__restric
     }
 }
 
+// Stable sort of the first m = min(n, *n_dev) keys ([kw][n] words, the most
+// significant word last, as the radix sort reads them) by rank: element i goes
+// to the number of keys below it plus the number of equal keys before it, the
+// position a stable sort gives it.  One workgroup, m^2 comparisons: for the
+// few runs of a nearly converged grid, where the radix sort's 12 launches
+// cost more than the sort.  Keys in LDS up to kSmallSortLds of them.
+constexpr int kSmallSortBlock = 1024;
+constexpr int kSmallSortLds = 1024;
+
+__global__ __launch_bounds__(kSmallSortBlock) void k_small_sort(
+    const uint64_t *__restrict__ kin, int n, int kw, const int *n_dev,
+    uint64_t *__restrict__ kout, int *__restrict__ vout, const DevState *st) {
+    if (st && st->done) return;
+    __shared__ uint64_t s_k[kMaxKW][kSmallSortLds];
+    const int m = min(n, *n_dev);
+    const bool lds = m <= kSmallSortLds;
+    if (lds) {
+        for (int e = threadIdx.x; e < m; e += blockDim.x)
+            for (int w = 0; w < kw; w++) s_k[w][e] = kin[(long long)w * n + e];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        uint64_t ki[kMaxKW];
+#pragma unroll
+        for (int w = 0; w < kMaxKW; w++)
+            ki[w] = w < kw ? (lds ? s_k[w][i] : kin[(long long)w * n + i]) : 0ull;
+        int rank = 0;
+        for (int j = 0; j < m; j++) {
+            int cmp = 0;  // key j against key i, most significant word first
+            for (int w = kw - 1; w >= 0 && cmp == 0; w--) {
+                const uint64_t a = lds ? s_k[w][j] : kin[(long long)w * n + j];
+                cmp = a < ki[w] ? -1 : (a > ki[w] ? 1 : 0);
+            }
+            rank += (cmp < 0 || (cmp == 0 && j < i)) ? 1 : 0;
+        }
+        for (int w = 0; w < kw; w++) kout[(long long)w * n + rank] = ki[w];
+        vout[rank] = i;
+    }
+}
+
+int small_sort(const uint64_t *keys_a, uint64_t *keys_b, int *vals_b, int n, int kw,
+               const int *n_dev, const DevState *st, hipStream_t s) {
+    k_small_sort<<<1, kSmallSortBlock, 0, s>>>(keys_a, n, kw, n_dev, keys_b, vals_b, st);
+    return 1;
+}
+
 size_t sort_tmp_ints(int n) {
     int ntiles = (n + kSortTile - 1) / kSortTile;
     if (ntiles == 0) ntiles = 1;

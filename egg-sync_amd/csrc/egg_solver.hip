@@ -438,11 +438,13 @@ double ms_between(hipEvent_t a, hipEvent_t b) {
 }
 
 // One pass of the loop body (EGG_SynC.cu:1323-1491), fully on the device.
+// small: the grid holds at most kSmallSortRuns runs (the grid build sorts
+// them in one launch).
 void enqueue_iteration(const Geo &g, Layout &L, const egg_sync_options &opt, hipStream_t s,
-                       hipEvent_t *ev) {
+                       hipEvent_t *ev, bool small = false) {
     if (ev) (void)hipEventRecord(ev[0], s);
     k_iter_begin<<<1, 1, 0, s>>>(L.st, L.it_before);
-    build_grid(g, L.b, L.st, L.st, s);
+    build_grid(g, L.b, L.st, L.st, s, small);
     if (opt.stop_rule == EGG_SYNC_STOP_EGG) cell_summaries(g, L.b, L.st, L.st, s);
     if (ev) (void)hipEventRecord(ev[1], s);
     if (opt.stop_rule == EGG_SYNC_STOP_EGG) {
@@ -567,21 +569,30 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
     }
     double st_build = 0, st_upd = 0, st_extra = 0;
     int done_iters = 0;
-    if (opt.use_graph) {
+    // runs only merge, so once a state read shows at most kSmallSortRuns
+    // valid runs, every later grid build may sort them in one launch
+    bool small = false;
+    auto ensure_graph = [&]() -> int {
+        if (!opt.use_graph) return 0;
         std::vector<char> key;
         key_add(key, ws->base);
         key_add(key, g);
         key_add(key, K);
         key_add(key, opt.stop_rule);
+        key_add(key, small);
         if (!ws->gexec || key != ws->gkey) {
             drop_graph(ws->gexec, ws->gkey);
-            rc = capture_graph(ws->stream, ws->gexec, [&] {
-                for (int k = 0; k < K; k++) enqueue_iteration(g, L, opt, ws->stream, nullptr);
+            const int r = capture_graph(ws->stream, ws->gexec, [&] {
+                for (int k = 0; k < K; k++)
+                    enqueue_iteration(g, L, opt, ws->stream, nullptr, small);
             });
-            if (rc) return rc;
+            if (r) return r;
             ws->gkey = key;
         }
-    }
+        return 0;
+    };
+    rc = ensure_graph();
+    if (rc) return rc;
     for (;;) {
         if (opt.use_graph) {
             if (opt.timing) (void)hipEventRecord(E.ev[0], s);
@@ -589,7 +600,7 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
             if (opt.timing) (void)hipEventRecord(E.ev[1], s);
         } else {
             for (int k = 0; k < K; k++)
-                enqueue_iteration(g, L, opt, s, opt.timing ? &E.ev[4 * k] : nullptr);
+                enqueue_iteration(g, L, opt, s, opt.timing ? &E.ev[4 * k] : nullptr, small);
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(h_state, L.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
@@ -616,13 +627,20 @@ int solve_device_impl(const float *d_D, int64_t n64, int d, float eps,
         }
         done_iters = h_state->iter;
         if (h_state->done) break;
+        const bool want = h_state->runs_valid && h_state->runs <= kSmallSortRuns;
+        if (want != small) {
+            small = want;
+            rc = ensure_graph();
+            if (rc) return rc;
+        }
     }
     const DevState loop_state = *h_state;
 
     // final grid on the converged positions and labels (EGG_SynC.cu:1504-1541)
     hipEvent_t *fe = opt.timing ? &E.ev[4 * K] : nullptr;
     if (fe) (void)hipEventRecord(fe[0], s);
-    build_grid(g, L.b, nullptr, L.st, s);
+    build_grid(g, L.b, nullptr, L.st, s,
+               loop_state.runs_valid && loop_state.runs <= kSmallSortRuns);
     int *labels_dev = d_labels ? d_labels : L.labels;
     if (opt.label_rule == EGG_SYNC_LABELS_CELLS)
         launch_labels_egg(g, L.b, labels_dev, L.b.nchunk, L.b.wi_off, L.st, s);
@@ -903,6 +921,9 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
     // instances still iterating; finished ones are no longer enqueued
     std::vector<char> live(n_inst, 0);
     for (int k : active) live[k] = 1;
+    // instances whose grid holds at most kSmallSortRuns runs (plain launches
+    // only: a captured graph keeps the radix sort)
+    std::vector<char> small(n_inst, 0);
     auto enqueue_batch = [&](hipStream_t origin) -> int {
         int r = fork_streams(bw, origin, S);
         if (r) return r;
@@ -910,7 +931,7 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
             for (int k : per[j])
                 if (live[k])
                     for (int it = 0; it < K; it++)
-                        enqueue_iteration(G[k], Ls[k], opt, bw.streams[j], nullptr);
+                        enqueue_iteration(G[k], Ls[k], opt, bw.streams[j], nullptr, small[k]);
         return join_streams(bw, origin, S);
     };
     // the graph holds the live set it was captured with; it is captured again
@@ -979,9 +1000,11 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
         HIP_TRY(hipStreamSynchronize(s));
         n_live = 0;
         for (int k : active) {
-            live[k] = !bw.h_states[k].done;
+            const DevState &hk = bw.h_states[k];
+            live[k] = !hk.done;
             n_live += live[k];
-            max_iter_seen = std::max(max_iter_seen, bw.h_states[k].iter);
+            max_iter_seen = std::max(max_iter_seen, hk.iter);
+            if (!opt.use_graph) small[k] = hk.runs_valid && hk.runs <= kSmallSortRuns;
         }
         if (n_live == 0) break;
         if (opt.use_graph && n_live != graph_live) {
@@ -997,7 +1020,8 @@ int solve_batch_impl(const float *d_D, const egg_sync_instance *inst, int n_inst
         hipStream_t sj = bw.streams[j];
         for (int k : per[j]) {
             Layout &L = Ls[k];
-            build_grid(G[k], L.b, nullptr, L.st, sj);
+            const DevState &hk = bw.h_states[k];
+            build_grid(G[k], L.b, nullptr, L.st, sj, hk.runs_valid && hk.runs <= kSmallSortRuns);
             int32_t *lab = d_labels + row_off[k];
             if (opt.label_rule == EGG_SYNC_LABELS_CELLS)
                 launch_labels_egg(G[k], L.b, lab, L.b.nchunk, L.b.wi_off, L.st, sj);
