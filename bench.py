@@ -176,18 +176,20 @@ def plain_d8_form():
             "clusters": r["stats"]["n_clusters"]}
 
 
-def batched_form(count=64):
-    """256 solves (64 datasets x 4 eps) in one batched call vs one call each."""
+def batched_form(count=64, streams=8):
+    """256 solves (64 datasets x 4 eps) in one batched call vs one call each.  The
+    batch deals its instances over `streams` HIP streams (8: the fastest of 1, 4
+    and 8 measured on this workload)."""
     import torch
     from egg_sync import data, solver
     inst = data.config3_instances(seed=3, count=count)
     D, specs = solver.pack_instances(inst, [list(data.CONFIG3_EPS)] * len(inst))
     print(f"batched form: {len(specs)} solves, warm-up", file=sys.stderr, flush=True)
-    solver.solve_batch(D, specs, return_positions=False)
+    solver.solve_batch(D, specs, streams=streams, return_positions=False)
     torch.cuda.synchronize()
     print("batched form: timed batch", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    r = solver.solve_batch(D, specs, return_positions=False)
+    r = solver.solve_batch(D, specs, streams=streams, return_positions=False)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     updates = sum(int(s[1]) * res["iterations"] for s, res in zip(specs, r["results"]))
@@ -202,7 +204,8 @@ def batched_form(count=64):
     t_single = time.perf_counter() - t0
     return {"workload": f"{len(specs)} solves: {count} datasets (n log-uniform in [1e3,2e5], "
                         "d in {2,3,5,8}) x eps {0.01,0.02,0.05,0.1}",
-            "points": int(sum(int(s[1]) for s in specs)), "batch_s": t_batch,
+            "points": int(sum(int(s[1]) for s in specs)), "streams": streams,
+            "batch_s": t_batch,
             "point_updates_per_s": updates / t_batch, "one_call_each_s": t_single,
             "speedup_vs_one_call_each": t_single / t_batch,
             "labels_equal_to_single_calls": singles_same,
