@@ -13,7 +13,8 @@
 // reference), pruned in the remaining dimensions to |dg| <= R+1 by a binary
 // search over the column's cells (sorted by g_od): cells farther than that
 // fail the reference's box test by more than a cell side, so results do not
-// change.
+// change.  Up to d = 3 the masked sums over streamed runs run on the matrix
+// cores (stream_chunk_mfma), in the same order of FP32 additions.
 #include <algorithm>
 #include <cmath>
 
@@ -330,6 +331,129 @@ __device__ __forceinline__ int stream_total(int r0, int r1, int s0, unsigned sma
     return total;
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// stream_chunk for d <= 3 with the masked sums on the matrix cores.  Each
+// source q is one step: the lanes test it against eps (the same FP32
+// distance as below), and two v_mfma_f32_4x4x1_16b add the row
+// [w sv_0..2 | w | w cv_0..2 | 0] of q (w = run length) times each lane's
+// 0/1 test result into 64 x 8 accumulators (16 blocks of 4 targets x 4
+// columns per instruction: block b holds targets 4(b&7)+i, +32 in the second
+// instruction, and columns 4(b>>3)+j).  K = 1, so every accumulator takes one
+// FP32 add per source, of the value or of 0, in source order: the sums are
+// those of the scalar loop below bit for bit, the VALU is spared the six
+// masked adds and the count per source.  A lane's sums enter the accumulator
+// layout through LDS before the chunk and leave it after.  The count column
+// is exact while a chunk's count stays below 2^24, hence n < 2^24.
+template <int D>
+__device__ __forceinline__ void stream_chunk_mfma(int base, int total, int r0, int s0,
+                                                  unsigned smask, const int *__restrict__ sub_run,
+                                                  const int *__restrict__ run_start,
+                                                  const float *__restrict__ Xs, int n,
+                                                  const float (&a0)[D], const float (&x)[D],
+                                                  float thr, bool acc, SrcPt<D> *src,
+                                                  float (&ssv)[D], float (&scm)[D], int &m_part) {
+    static_assert(D <= 3 && SrcPt<D>::P == 4, "row of a source: sv[0..3] then cv[0..3]");
+    const int lane = __lane_id();
+    float *T = reinterpret_cast<float *>(src);  // 64 x 8 transpose buffer, inside src
+    // the accumulators this lane holds: targets tc + r (and 32 + tc + r), column cc
+    const int tc = lane & 28, cc = 4 * (lane >> 5) + (lane & 3);
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+        T[lane * 8 + l] = l < D ? ssv[l] : 0.f;
+        T[lane * 8 + 4 + l] = l < D ? scm[l] : 0.f;
+    }
+    T[lane * 8 + 3] = 0.f;  // count column: this chunk's count only
+    wave_sync();
+    f32x4 d1, d2;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        d1[r] = T[(tc + r) * 8 + cc];
+        d2[r] = T[(32 + tc + r) * 8 + cc];
+    }
+    wave_sync();
+    const int rr = stream_run(base + lane, total, r0, s0, smask, sub_run);
+    if (rr >= 0) {
+        const int j = run_start[rr];
+        const float w = (float)(run_start[rr + 1] - j);  // exact: < 2^24
+        float sv[D], cv[D];
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            const float xq = Xs[(long long)l * n + j];
+            src[lane].x[l] = xq;
+            sincosm1_small(xq - a0[l], sv[l], cv[l]);
+        }
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            src[lane].sv[l] = l < D ? w * sv[l] : 0.f;
+            src[lane].cv[l] = l < D ? w * cv[l] : 0.f;
+        }
+        src[lane].sv[3] = w;
+    }
+    wave_sync();
+    const int m = __builtin_amdgcn_readfirstlane(min(kWave, total - base));
+    const float thr_l = acc ? thr : -1.f;  // lanes without acc add zeros
+    constexpr int kS = sizeof(SrcPt<D>) / sizeof(float);
+    const float *row = src[0].sv;  // row[q * kS + c]: column c of source q
+    // per source: the lanes' 0/1 test results, lanes 0-31's to both halves of
+    // the first MFMA's operand (targets 0-31) and 32-63's to the second's
+    int k = 0;
+    for (; k + 4 <= m; k += 4) {  // four sources per step: their LDS reads first
+        float q[4][D], b[4], a[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+#pragma unroll
+            for (int l = 0; l < D; l++) q[jj][l] = src[k + jj].x[l];
+            b[jj] = row[(k + jj) * kS + cc];
+        }
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            float sq = 0.f;
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                const float df = x[l] - q[jj][l];
+                sq += df * df;
+            }
+            a[jj] = sq <= thr_l ? 1.f : 0.f;
+        }
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            const auto pr = __builtin_amdgcn_permlane32_swap(__float_as_uint(a[jj]),
+                                                             __float_as_uint(a[jj]), false, false);
+            d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(__uint_as_float(pr[0]), b[jj], d1, 0, 0, 0);
+            d2 = __builtin_amdgcn_mfma_f32_4x4x1f32(__uint_as_float(pr[1]), b[jj], d2, 0, 0, 0);
+        }
+    }
+    for (; k < m; k++) {
+        float sq = 0.f;
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            const float df = x[l] - src[k].x[l];
+            sq += df * df;
+        }
+        const float a = sq <= thr_l ? 1.f : 0.f;
+        const auto pr =
+            __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(a), false, false);
+        const float b = row[k * kS + cc];
+        d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(__uint_as_float(pr[0]), b, d1, 0, 0, 0);
+        d2 = __builtin_amdgcn_mfma_f32_4x4x1f32(__uint_as_float(pr[1]), b, d2, 0, 0, 0);
+    }
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        T[(tc + r) * 8 + cc] = d1[r];
+        T[(32 + tc + r) * 8 + cc] = d2[r];
+    }
+    wave_sync();
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        ssv[l] = T[lane * 8 + l];
+        scm[l] = T[lane * 8 + 4 + l];
+    }
+    m_part += (int)T[lane * 8 + 3];
+    wave_sync();
+}
+
 // One chunk of a streamed range: runs [base, base + 64) of it go through
 // LDS (each lane loads one run and computes sin v and cos v - 1 of its
 // offset v from the anchor a0); lanes with `acc` test each run against eps
@@ -344,6 +468,13 @@ __device__ __forceinline__ void stream_chunk(int base, int total, int r0, int s0
                                              float thr, bool acc, bool any_acc, SrcPt<D> *src,
                                              int *len, float (&ssv)[D], float (&scm)[D],
                                              int &m_part) {
+    if constexpr (D <= 3) {
+        if (any_acc && n < (1 << 24)) {
+            stream_chunk_mfma<D>(base, total, r0, s0, smask, sub_run, run_start, Xs, n, a0, x, thr,
+                                 acc, src, ssv, scm, m_part);
+            return;
+        }
+    }
     const int lane = __lane_id();
     const int rr = stream_run(base + lane, total, r0, s0, smask, sub_run);
     int my_len = 1;
