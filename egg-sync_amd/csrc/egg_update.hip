@@ -355,21 +355,26 @@ __device__ __forceinline__ void stream_chunk_mfma(int base, int total, int r0, i
                                                   float (&ssv)[D], float (&scm)[D], int &m_part) {
     static_assert(D <= 3 && SrcPt<D>::P == 4, "row of a source: sv[0..3] then cv[0..3]");
     const int lane = __lane_id();
-    float *T = reinterpret_cast<float *>(src);  // 64 x 8 transpose buffer, inside src
+    // 64 x 8 transpose buffer inside src, rows of kTr = 9 floats: an odd
+    // stride puts a row-wise access of 32 lanes in 32 different banks, and
+    // the accumulator-layout access too (9 * 4k + j covers 0..31 mod 32)
+    constexpr int kTr = 9;
+    static_assert(kWave * kTr <= kWave * (int)(sizeof(SrcPt<D>) / sizeof(float)), "T fits in src");
+    float *T = reinterpret_cast<float *>(src);
     // the accumulators this lane holds: targets tc + r (and 32 + tc + r), column cc
     const int tc = lane & 28, cc = 4 * (lane >> 5) + (lane & 3);
 #pragma unroll
     for (int l = 0; l < 4; l++) {
-        T[lane * 8 + l] = l < D ? ssv[l] : 0.f;
-        T[lane * 8 + 4 + l] = l < D ? scm[l] : 0.f;
+        T[lane * kTr + l] = l < D ? ssv[l] : 0.f;
+        T[lane * kTr + 4 + l] = l < D ? scm[l] : 0.f;
     }
-    T[lane * 8 + 3] = 0.f;  // count column: this chunk's count only
+    T[lane * kTr + 3] = 0.f;  // count column: this chunk's count only
     wave_sync();
     f32x4 d1, d2;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        d1[r] = T[(tc + r) * 8 + cc];
-        d2[r] = T[(32 + tc + r) * 8 + cc];
+        d1[r] = T[(tc + r) * kTr + cc];
+        d2[r] = T[(32 + tc + r) * kTr + cc];
     }
     wave_sync();
     const int rr = stream_run(base + lane, total, r0, s0, smask, sub_run);
@@ -441,16 +446,16 @@ __device__ __forceinline__ void stream_chunk_mfma(int base, int total, int r0, i
     wave_sync();
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        T[(tc + r) * 8 + cc] = d1[r];
-        T[(32 + tc + r) * 8 + cc] = d2[r];
+        T[(tc + r) * kTr + cc] = d1[r];
+        T[(32 + tc + r) * kTr + cc] = d2[r];
     }
     wave_sync();
 #pragma unroll
     for (int l = 0; l < D; l++) {
-        ssv[l] = T[lane * 8 + l];
-        scm[l] = T[lane * 8 + 4 + l];
+        ssv[l] = T[lane * kTr + l];
+        scm[l] = T[lane * kTr + 4 + l];
     }
-    m_part += (int)T[lane * 8 + 3];
+    m_part += (int)T[lane * kTr + 3];
     wave_sync();
 }
 
