@@ -39,11 +39,13 @@ ITERS_PER_SYNC = 4
 # `bench.py --steps 1 --warmup 0 --no-cpu --forms plain` (mean over the solve's
 # passes; the pass is the kernels listed under "kernel" below): HBM bytes
 # (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a copy
-# of known size) and VALU wave instructions (SQ_INSTS_VALU).  Measured for
-# this workload only, so reported only when n is 1,000,000.
+# of known size) and VALU wave instructions (SQ_INSTS_VALU, which counts the
+# pass's MFMA instructions too; those alone are UPDATE_MFMA_INSTS_N1M).
+# Measured for this workload only, so reported only when n is 1,000,000.
 UPDATE_PROFILE = "profiles/update_pass_config1_n1M_counters.md"
-UPDATE_TRAFFIC_BYTES_N1M = 223.65e6
-UPDATE_VALU_INSTS_N1M = 4.25e9
+UPDATE_TRAFFIC_BYTES_N1M = 826.7e6
+UPDATE_VALU_INSTS_N1M = 4.15e9
+UPDATE_MFMA_INSTS_N1M = 2.75e8
 
 
 def cpu_baseline(X, eps, target_seconds=12.0):
@@ -402,7 +404,8 @@ def main():
         valu = UPDATE_VALU_INSTS_N1M / avg_launch_s
         line["roofline"]["valu"] = {"achieved": valu, "peak": VALU_PEAK, "unit": "wave-instr/s",
                                     "frac": valu / VALU_PEAK, "instructions_per_launch":
-                                    UPDATE_VALU_INSTS_N1M, "source": UPDATE_PROFILE}
+                                    UPDATE_VALU_INSTS_N1M, "mfma_instructions_per_launch":
+                                    UPDATE_MFMA_INSTS_N1M, "source": UPDATE_PROFILE}
     try:
         cp = copy_bandwidth()
         line["roofline"]["copy_gbs_measured"] = cp
