@@ -178,10 +178,10 @@ def plain_d8_form():
             "clusters": r["stats"]["n_clusters"]}
 
 
-def batched_form(count=64, streams=8):
+def batched_form(count=64, streams=24):
     """256 solves (64 datasets x 4 eps) in one batched call vs one call each.  The
-    batch deals its instances over `streams` HIP streams (8: the fastest of 1, 4
-    and 8 measured on this workload)."""
+    batch deals its instances over `streams` HIP streams (24: the fastest of 1, 4,
+    8, 12, 16, 24, 32, 48 and 64 measured on this workload)."""
     import torch
     from egg_sync import data, solver
     inst = data.config3_instances(seed=3, count=count)

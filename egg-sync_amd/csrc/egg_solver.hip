@@ -749,7 +749,7 @@ struct BatchWs {
 
 BatchWs g_bws[64];
 
-constexpr int kMaxStreams = 16;
+constexpr int kMaxStreams = 64;
 
 int batch_resources(BatchWs &bw, size_t bytes, int n_inst, int S) {
     if (bw.bytes < bytes) {

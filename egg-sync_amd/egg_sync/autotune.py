@@ -33,7 +33,7 @@ PLAIN_SPACE = {
 }
 
 BATCH_SPACE = {
-    "streams": [4, 1, 2, 8],
+    "streams": [4, 1, 2, 8, 16, 24, 32],
     "use_graph": [False, True],
     "iters_per_sync": [4, 2, 8],
     "queue_threads": [256, 128],

@@ -73,7 +73,8 @@ typedef struct egg_sync_options {
     int32_t queue_blocks;    /* workgroups of the tile-queue kernels (0 = auto)   */
     int32_t queue_threads;   /* threads per such workgroup: 64, 128, 192 or 256
                                 (0 = 256)                                         */
-    int32_t streams;         /* batched solve: concurrent HIP streams (0 = 4)     */
+    int32_t streams;         /* batched solve: concurrent HIP streams, at most 64
+                                (0 = 4)                                           */
     int32_t tile_group;      /* update tiles: neighbouring cells that differ only
                                 in the last coordinate, within aligned blocks of
                                 this many cells (1, 2, 4 or 8; others round down),

@@ -89,7 +89,8 @@ def _mixed_instances():
     return sets, eps
 
 
-@pytest.mark.parametrize("streams,graph", [(1, False), (3, False), (4, True), (2, True)])
+@pytest.mark.parametrize("streams,graph", [(1, False), (3, False), (4, True), (2, True),
+                                           (24, False), (24, True)])
 def test_batch_equals_single_solves(S, streams, graph):
     sets, eps = _mixed_instances()
     D, specs = S.pack_instances(sets, eps)
