@@ -339,7 +339,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // [w sv_0..2 | w | w cv_0..2 | 0] of q (w = run length) times each lane's
 // 0/1 test result into 64 x 8 accumulators (16 blocks of 4 targets x 4
 // columns per instruction: block b holds targets 4(b&7)+i, +32 in the second
-// instruction, and columns 4(b>>3)+j).  K = 1, so every accumulator takes one
+// instruction, and columns 4(b>>3)+j; lane l supplies row l&3 of block l>>2's
+// A and column l&3 of its B, and holds that block's rows 0-3 of column l&3 in
+// its four result registers).  K = 1, so every accumulator takes one
 // FP32 add per source, of the value or of 0, in source order: the sums are
 // those of the scalar loop below bit for bit, the VALU is spared the six
 // masked adds and the count per source.  A lane's sums enter the accumulator
